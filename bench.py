@@ -1,0 +1,198 @@
+"""Benchmark of the PBF step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload: the 1M-particle dam break (h = 0.02, box 2.027 x 2.027 x 1.0135,
+102 x 102 x 51 = 530,604 cells, 4 solver iterations, dt = 1/120), seeded
+with PCG32 jitter; state stays resident in HBM for the whole timed region.
+
+One JSON line on stdout:
+  value        algorithmic HBM GB/s of the whole step, B_step / t_step with
+               B_step = (100 + 44 I) N + (16 + 8 I) C bytes (the compulsory
+               traffic of every stage: vec3 = 12 B, u32/f32 = 4 B)
+  ms_per_step  wall time of K steps between two device synchronisations / K
+  roofline     the dominant kernel: its algorithmic bytes per launch over its
+               mean duration from HIP events recorded on the launch stream
+               during the timed steps; traffic = HBM bytes per launch from
+               rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/), else null
+  cpu_baseline the oracle's FP32 step on this host's cores (OpenMP), timed
+               on a bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+N_PARTICLES = 1_000_000
+SCENE_DAM = 4
+DT = 1.0 / 120.0
+
+
+def stage_bytes(name, n, c):
+    """Algorithmic (compulsory) bytes of one launch of each stage."""
+    return {
+        "predict_hash_count": 28 * n + 4 * c,   # R x,v (24N), W key (4N), counts (4C)
+        "exclusive_scan": 8 * c,                 # R counts + W cell_start
+        "stable_scatter_gather": 20 * n + 4 * c, # R key, cell_start; W index, sorted x
+        "density_lambda": 16 * n + 4 * c,        # R sorted x (12N), cell_start; W lambda
+        "apply_delta": 28 * n + 4 * c,           # R sorted x + lambda (16N), cell_start; W x'
+        "scatter_finalize": 52 * n,              # R x', index, prev; W x, v
+        "count_neighbors": 16 * n + 4 * c,
+    }.get(name, 0)
+
+
+def step_bytes(n, c, iters):
+    return (100 + 44 * iters) * n + (16 + 8 * iters) * c
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from profiles/hbm_traffic.txt
+    (lines: `<kernel> <workload> fetch_bytes=<B> write_bytes=<B>`)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 4 and parts[0] == kernel and parts[1] == "dam_break_1m":
+                kv = dict(p.split("=", 1) for p in parts[2:] if "=" in p)
+                try:
+                    return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+                except (KeyError, ValueError):
+                    return None
+    return None
+
+
+def cpu_baseline(x, v, params, iters_budget_s=20.0):
+    """The oracle's FP32 step on the host (all cores), on whole 1M steps."""
+    from oracle import build as obuild
+    from oracle import oracle as O
+    if not os.path.exists(O.LIB_PATH):
+        obuild.build(force=True)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    O.set_threads(threads)
+    op = O.params(params)
+    g = O.grid(op)
+    O.step(x, v, op, DT, prec=32)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 and time.perf_counter() - t_start < iters_budget_s:
+        t0 = time.perf_counter()
+        O.step(x, v, op, DT, prec=32)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    b = step_bytes(len(x), g.total_cells, op.iterations)
+    return {"value": round(b / t / 1e9, 3), "unit": "GB/s", "cores": O.get_threads(), "kind": "port",
+            "ms_per_step": round(t * 1e3, 1),
+            "sample": f"{len(times)} whole FP32 oracle steps (after 1 warm-up) of the same 1M dam break, "
+                      f"median; OpenMP over particles, serial scan/scatter"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); "
+              "a multi-GPU figure is not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mtlphys_amd as mtlp
+
+    params = mtlp.scene_params(SCENE_DAM, N_PARTICLES)
+    x, v = mtlp.generate_scene(SCENE_DAM, N_PARTICLES, 1, params)
+    iters = params.iterations
+    eng = mtlp.Engine(0, params)
+    eng.reset(N_PARTICLES, mtlp.capi.SCENE_EMPTY)
+    eng.upload_state(x, v)
+    cells = eng.total_cells
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        eng.step(DT, stream)
+    torch.cuda.synchronize()
+
+    # timed region: K steps, HIP events after every kernel on the launch stream
+    eng.set_profiling(True)
+    per_stage = {}
+    step_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.step(DT, stream)
+        st = eng.stage_times()  # waits for this step's last event
+        step_ms.append(sum(ms for _, ms in st))
+        for name, ms in st:
+            per_stage.setdefault(name, []).append(ms)
+    torch.cuda.synchronize()
+    t_prof = (time.perf_counter() - t0) / args.steps
+    eng.set_profiling(False)
+
+    # same K steps without events or host waits between steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.step(DT, stream)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.steps
+    flags = eng.error_flags()
+    xs = eng.positions_buffer().read()
+    eng.close()
+    if flags or not np.isfinite(xs).all():
+        print(f"bench.py: device error flags {flags:#x} / non-finite state", file=sys.stderr)
+        sys.exit(1)
+
+    b_step = step_bytes(N_PARTICLES, cells, iters)
+    kernels = {}
+    for name, ms in per_stage.items():
+        mean_ms = statistics.fmean(ms)
+        b = stage_bytes(name, N_PARTICLES, cells)
+        kernels[name] = {"launches_per_step": len(ms) // args.steps, "mean_us": round(mean_ms * 1e3, 2),
+                         "alg_bytes": b, "gbs": round(b / (mean_ms * 1e-3) / 1e9, 1) if mean_ms > 0 else None}
+    dom = max(kernels, key=lambda k: kernels[k]["mean_us"] * kernels[k]["launches_per_step"])
+    dk = kernels[dom]
+    achieved = dk["gbs"]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": read_traffic(dom),
+                "alg_bytes_per_launch": dk["alg_bytes"], "mean_us": dk["mean_us"]}
+
+    out = {
+        "metric": "PBF step ms + HBM GB/s at 1M particles (dam break, 4 iterations)",
+        "value": round(b_step / t_step / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "ms_per_step_p50": round(statistics.median(step_ms), 4),
+        "ms_per_step_p95": round(float(np.percentile(step_ms, 95)), 4),
+        "ms_per_step_with_events": round(t_prof * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: PCG32-jittered lattice dam break, seed 1",
+        "config": {"workload": "dam_break_1m", "particles": N_PARTICLES, "h": params.h, "cells": cells,
+                   "grid": list(mtlp.grid_dims(params)), "iterations": iters, "dt": DT,
+                   "alg_bytes_per_step": b_step},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(x, v, params),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

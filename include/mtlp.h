@@ -1,0 +1,187 @@
+/*
+ * mtlp.h -- C ABI of the MI355X (gfx950) PBF engine.
+ *
+ * Drop-in boundary for the reference's `class mtlphys::Engine`
+ * (engine/include/mtlphys/Engine.hpp:27-100).  Every entry point below
+ * names the reference member it replaces.  Conventions:
+ *
+ *   ownership  The engine owns its device buffers.  mtlp_get_buffer returns
+ *              borrowed device pointers that stay valid (same address) until
+ *              mtlp_reset or mtlp_destroy, like the reference's
+ *              MTL::Buffer accessors (Engine.hpp:83-90).  Scan scratch is
+ *              caller-owned, like runExclusiveScan's chunkSums
+ *              (Engine.hpp:96-100).
+ *   layout     Positions and velocities are float4 AoS: pos.w = 1,
+ *              vel.w = 0 (Constraints.metal:351-352).  Buffers are padded
+ *              to a multiple of 256 elements (Engine.cpp:383).
+ *   streams    Work is enqueued on the caller's stream (a hipStream_t passed
+ *              as mtlp_stream; NULL = the null stream), the analogue of the
+ *              caller's MTL::CommandBuffer (Engine.hpp:52-56).  Calls return
+ *              after enqueueing; the caller synchronises.  Only the
+ *              *_read_* / *_write_* / *_upload_* helpers and
+ *              mtlp_error_flags block.
+ *   errors     Status codes, never abort.  Conditions on which the reference
+ *              aborts (Engine.cpp:69-85, 407-413, 849-854) return
+ *              MTLP_E_RANGE / MTLP_E_HIP; its silent no-ops (dt <= 0,
+ *              Engine.cpp:686; n == 0, :848) return MTLP_OK.
+ *   threads    One engine per host thread (Engine.hpp:4-6).
+ */
+#ifndef MTLP_H
+#define MTLP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mtlp_engine mtlp_engine;
+typedef void* mtlp_stream; /* hipStream_t */
+
+typedef enum mtlp_status {
+    MTLP_OK = 0,
+    MTLP_E_INVALID = 1, /* null handle/pointer, bad enum, misaligned buffer  */
+    MTLP_E_RANGE = 2,   /* size beyond a limit (e.g. grid, scan length)      */
+    MTLP_E_NOMEM = 3,   /* device or host allocation failed                  */
+    MTLP_E_HIP = 4,     /* a HIP runtime call failed (see mtlp_last_error)   */
+    MTLP_E_DEVICE = 5   /* a kernel raised an error flag (mtlp_error_flags)  */
+} mtlp_status;
+
+/* Scene ids.  0-2 are the reference's (Engine.hpp:37-39, seeding at
+ * Engine.cpp:454-549).  3-4 are the benchmark workloads: a 32^3 cube drop
+ * at h = 0.1 and a 1M-particle dam break at h = 0.02, with seeded jitter
+ * (their parameters come from mtlp_scene_params). */
+enum {
+    MTLP_SCENE_CUBE_DROP = 0,
+    MTLP_SCENE_DAM_BREAK = 1,
+    MTLP_SCENE_SPHERE_DROP = 2, /* the lake only; the sphere is not modelled */
+    MTLP_SCENE_JITTER_CUBE = 3,
+    MTLP_SCENE_JITTER_DAM = 4,
+    MTLP_SCENE_EMPTY = 255      /* positions and velocities zeroed           */
+};
+
+/* Simulation parameters.  mtlp_default_params gives the reference values:
+ * Engine.cpp:23-39 (bounds, cell size, h, rho0, epsilon, damping,
+ * iterations), Engine.cpp:558 (gravity), Constraints.metal:158-160
+ * (s_corr k, n, dq), :190 (0.3 h step cap), :325 (wall band), :340-345
+ * (friction), :347 (speed cap h/dt). */
+typedef struct mtlp_params {
+    float gravity[3];
+    float bounds_min[3];
+    float bounds_max[3];
+    float h;              /* smoothing length                               */
+    float cell_size;      /* hash cell edge (the reference uses h)          */
+    float rest_density;
+    float epsilon;        /* constraint-force mixing                       */
+    float scorr_k;
+    float scorr_dq;       /* delta q / h                                    */
+    float max_step;       /* |delta x| cap / h                              */
+    float damping;        /* velocity factor at finalize                    */
+    float wall_eps;       /* wall band width                                */
+    float floor_friction; /* tangential factor on the floor                 */
+    float wall_friction;  /* factor on side walls                           */
+    float vmax;           /* speed cap / (h/dt)                             */
+    uint32_t scorr_n;     /* s_corr exponent                                */
+    uint32_t iterations;  /* solver iterations per step                     */
+} mtlp_params;
+
+/* Buffer ids = the reference accessors (Engine.hpp:83-90), plus the
+ * sorted-order positions of the last step (gatherPositions' output,
+ * Spatial.metal:190-198, holding the final solver iterate after a step). */
+typedef enum mtlp_buffer {
+    MTLP_BUF_POSITIONS = 0,       /* float4[n]                              */
+    MTLP_BUF_VELOCITIES = 1,      /* float4[n]                              */
+    MTLP_BUF_PREV_POSITIONS = 2,  /* float4[n], positions before the step   */
+    MTLP_BUF_CELL_HASHES = 3,     /* uint32[n]                              */
+    MTLP_BUF_CELL_START = 4,      /* uint32[total_cells + 1]                */
+    MTLP_BUF_SORTED_INDEX = 5,    /* uint32[n]                              */
+    MTLP_BUF_NEIGHBOR_COUNTS = 6, /* uint32[n]                              */
+    MTLP_BUF_LAMBDAS = 7,         /* float[n], last iteration, orig. order  */
+    MTLP_BUF_SORTED_POSITIONS = 8 /* float4[n]; .w of a step's final iterate
+                                     holds that iterate's lambda input     */
+} mtlp_buffer;
+
+/* Largest scan length (Engine.hpp:93 caps the reference at 1024^2; the
+ * single-pass scan here has no such cap below 2^31). */
+#define MTLP_MAX_SCAN_LENGTH 0x7fffffffu
+/* Largest particle count per engine. */
+#define MTLP_MAX_PARTICLES 0x40000000u
+
+const char* mtlp_status_string(mtlp_status s);
+/* Message of the last failing call on this thread ("" if none). */
+const char* mtlp_last_error(void);
+
+mtlp_status mtlp_default_params(mtlp_params* out);
+/* Parameters a scene id is defined with (defaults for scenes 0-2). */
+mtlp_status mtlp_scene_params(uint32_t scene, uint32_t n, mtlp_params* out);
+/* Host-only scene generator (Engine::seedParticles, Engine.cpp:454-549):
+ * fills x4/v4 (4n floats each).  Needs no GPU. */
+mtlp_status mtlp_generate_scene(uint32_t scene, uint32_t n, uint64_t seed,
+                                const mtlp_params* p, float* x4, float* v4);
+/* Grid of a parameter set (Engine::buildSpatialBuffers, Engine.cpp:398-413):
+ * dims = ceil((bounds_max - bounds_min) / cell_size) per axis.  Host only. */
+mtlp_status mtlp_grid_dims(const mtlp_params* p, uint32_t dims[3]);
+
+/* Engine(MTL::Device*) (Engine.cpp:97-102).  Allocates nothing until
+ * mtlp_reset. */
+mtlp_status mtlp_create(int hip_device, const mtlp_params* p, mtlp_engine** out);
+/* ~Engine (Engine.cpp:104-123) */
+mtlp_status mtlp_destroy(mtlp_engine* e);
+/* Engine::reset(n, scene) (Engine.cpp:129-134): (re)allocates every buffer
+ * and seeds the scene; synchronous. */
+mtlp_status mtlp_reset(mtlp_engine* e, uint32_t n, uint32_t scene, uint64_t seed);
+/* Replaces the parameters (the reference writes PBFParams every step,
+ * Engine.cpp:551-572).  A change of bounds or cell size that changes the
+ * grid reallocates the cell buffers; synchronous. */
+mtlp_status mtlp_set_params(mtlp_engine* e, const mtlp_params* p);
+mtlp_status mtlp_get_params(const mtlp_engine* e, mtlp_params* out);
+/* Replaces the reference's host writes into the Shared position and
+ * velocity buffers (tests/test_pbf.cpp:197-200).  Either pointer may be
+ * NULL.  Synchronous. */
+mtlp_status mtlp_upload_state(mtlp_engine* e, const float* x4, const float* v4, uint32_t n);
+
+/* Engine::step(cmd, dt) (Engine.cpp:685-811): predict, spatial hash,
+ * iterations x (density/lambda, delta), scatter back, finalize.  dt <= 0 is
+ * a no-op returning MTLP_OK. */
+mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt);
+/* Engine::buildSpatialHash(cmd) (Engine.cpp:601-679) on the current
+ * positions: hash, count, scan, stable scatter, gather, neighbour count. */
+mtlp_status mtlp_build_spatial_hash(mtlp_engine* e, mtlp_stream stream);
+
+/* Engine::runExclusiveScan(cmd, in, out, chunkSums, n) (Engine.cpp:843-882).
+ * d_in and d_out are device pointers, 16-byte aligned, n elements; d_scratch
+ * holds mtlp_scan_scratch_bytes(n) bytes.  n == 0 is a no-op; the engine may
+ * be NULL. */
+size_t mtlp_scan_scratch_bytes(uint32_t n);
+mtlp_status mtlp_exclusive_scan(mtlp_engine* e, mtlp_stream stream, const uint32_t* d_in,
+                                uint32_t* d_out, void* d_scratch, uint32_t n);
+
+/* Accessors (Engine.hpp:78-90). */
+mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, size_t* bytes);
+/* Synchronous device->host / host->device copies of a buffer (the tests'
+ * readBuffer, tests/TestUtils.hpp:82-97).  bytes may be smaller than the
+ * buffer. */
+mtlp_status mtlp_read_buffer(mtlp_engine* e, mtlp_buffer which, void* host_dst, size_t bytes);
+mtlp_status mtlp_write_buffer(mtlp_engine* e, mtlp_buffer which, const void* host_src, size_t bytes);
+uint32_t mtlp_particle_count(const mtlp_engine* e);
+uint32_t mtlp_total_cells(const mtlp_engine* e);
+float mtlp_cell_size(const mtlp_engine* e);
+
+/* Diagnostics (no reference counterpart; the reference's only timing is
+ * GPUEndTime - GPUStartTime per command buffer, tests/bench_engine.cpp:43).
+ * With profiling on, mtlp_step records HIP events on its stream around each
+ * kernel; mtlp_stage_times synchronises and returns the last call's kernel
+ * durations in ms with their names. */
+mtlp_status mtlp_set_profiling(mtlp_engine* e, int enable);
+mtlp_status mtlp_stage_times(mtlp_engine* e, float* ms, const char** names,
+                             uint32_t capacity, uint32_t* count);
+/* Device error flags raised by kernels since the last call (bit 0: scan
+ * look-back gave up waiting); synchronises and clears them. */
+mtlp_status mtlp_error_flags(mtlp_engine* e, uint32_t* flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTLP_H */

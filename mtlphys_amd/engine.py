@@ -1,0 +1,225 @@
+"""Python mirror of the reference engine interface (engine/include/mtlphys/Engine.hpp).
+
+`Engine` wraps the C ABI of libmtlp.so with the reference's member names:
+reset, step, build_spatial_hash, run_exclusive_scan, the buffer accessors,
+particle_count, total_cells and cell_size.  Every GPU computation runs in the
+HIP kernels of libmtlp.so; this module only moves handles and host arrays.
+
+Streams: methods take `stream=None` (the null stream), an integer hipStream_t
+handle, or any object with a `.cuda_stream` attribute (a torch stream).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi as C
+
+# buffer id -> (numpy dtype, components per particle / cell)
+_BUFFER_LAYOUT = {
+    C.BUF_POSITIONS: (np.float32, 4),
+    C.BUF_VELOCITIES: (np.float32, 4),
+    C.BUF_PREV_POSITIONS: (np.float32, 4),
+    C.BUF_CELL_HASHES: (np.uint32, 1),
+    C.BUF_CELL_START: (np.uint32, 1),
+    C.BUF_SORTED_INDEX: (np.uint32, 1),
+    C.BUF_NEIGHBOR_COUNTS: (np.uint32, 1),
+    C.BUF_LAMBDAS: (np.float32, 1),
+    C.BUF_SORTED_POSITIONS: (np.float32, 4),
+}
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(int(stream.cuda_stream))
+    return ctypes.c_void_p(int(stream))
+
+
+def _ptr(arr):
+    return arr.ctypes.data_as(ctypes.c_void_p)
+
+
+def generate_scene(scene, n, seed=0, params=None):
+    """Host-side scene seeding (Engine::seedParticles, Engine.cpp:454-549).
+
+    Returns (x4, v4) as float32 arrays of shape (n, 4).  Needs no GPU."""
+    p = params if params is not None else C.scene_params(scene, n)
+    x = np.zeros((n, 4), np.float32)
+    v = np.zeros((n, 4), np.float32)
+    C.check(C.lib().mtlp_generate_scene(scene, n, seed, ctypes.byref(p), _ptr(x), _ptr(v)),
+            "mtlp_generate_scene")
+    return x, v
+
+
+class DeviceBuffer:
+    """Borrowed view of one engine buffer (the reference's MTL::Buffer* accessors)."""
+
+    def __init__(self, engine, which):
+        self._engine = engine
+        self.which = which
+        ptr = ctypes.c_void_p()
+        nbytes = ctypes.c_size_t()
+        C.check(C.lib().mtlp_get_buffer(engine._h, which, ctypes.byref(ptr), ctypes.byref(nbytes)),
+                "mtlp_get_buffer")
+        self.ptr = ptr.value or 0
+        self.nbytes = nbytes.value
+        self.dtype, self.components = _BUFFER_LAYOUT[which]
+
+    def read(self):
+        """Synchronous device->host copy (tests/TestUtils.hpp:82-97)."""
+        itemsize = np.dtype(self.dtype).itemsize * self.components
+        count = self.nbytes // itemsize
+        shape = (count, self.components) if self.components > 1 else (count,)
+        out = np.empty(shape, self.dtype)
+        C.check(C.lib().mtlp_read_buffer(self._engine._h, self.which, _ptr(out), out.nbytes),
+                "mtlp_read_buffer")
+        return out
+
+    def write(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        C.check(C.lib().mtlp_write_buffer(self._engine._h, self.which, _ptr(arr), arr.nbytes),
+                "mtlp_write_buffer")
+
+
+class Engine:
+    """mtlphys::Engine over libmtlp.so (Engine.hpp:27-100)."""
+
+    kSceneCubeDrop = C.SCENE_CUBE_DROP
+    kSceneDamBreak = C.SCENE_DAM_BREAK
+    kSceneSphereDrop = C.SCENE_SPHERE_DROP
+
+    def __init__(self, device=0, params=None):
+        self._h = None
+        p = params if params is not None else C.default_params()
+        h = ctypes.c_void_p()
+        C.check(C.lib().mtlp_create(int(device), ctypes.byref(p), ctypes.byref(h)), "mtlp_create")
+        self._h = h
+        self._scene = C.SCENE_CUBE_DROP
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            C.check(C.lib().mtlp_destroy(self._h), "mtlp_destroy")
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- Engine.hpp:41-56 -------------------------------------------------
+    def reset(self, particle_count, scene=None, seed=0):
+        """Engine::reset(n[, scene]) (Engine.cpp:125-134)."""
+        if scene is None:
+            scene = self._scene
+        C.check(C.lib().mtlp_reset(self._h, int(particle_count), int(scene), int(seed)), "mtlp_reset")
+        self._scene = scene
+
+    def current_scene(self):
+        return self._scene
+
+    def step(self, dt, stream=None):
+        """Engine::step(cmd, dt) (Engine.cpp:685-811); asynchronous."""
+        C.check(C.lib().mtlp_step(self._h, _stream_handle(stream), ctypes.c_float(dt)), "mtlp_step")
+
+    def build_spatial_hash(self, stream=None):
+        """Engine::buildSpatialHash(cmd) (Engine.cpp:601-679); asynchronous."""
+        C.check(C.lib().mtlp_build_spatial_hash(self._h, _stream_handle(stream)),
+                "mtlp_build_spatial_hash")
+
+    def run_exclusive_scan(self, d_in, d_out, d_scratch, n, stream=None):
+        """Engine::runExclusiveScan (Engine.cpp:843-882) on device pointers."""
+        C.check(C.lib().mtlp_exclusive_scan(self._h, _stream_handle(stream), ctypes.c_void_p(d_in),
+                                            ctypes.c_void_p(d_out), ctypes.c_void_p(d_scratch), n),
+                "mtlp_exclusive_scan")
+
+    @staticmethod
+    def scan_scratch_bytes(n):
+        return C.lib().mtlp_scan_scratch_bytes(int(n))
+
+    # ---- parameters and state ----------------------------------------------
+    @property
+    def params(self):
+        p = C.Params()
+        C.check(C.lib().mtlp_get_params(self._h, ctypes.byref(p)), "mtlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        C.check(C.lib().mtlp_set_params(self._h, ctypes.byref(p)), "mtlp_set_params")
+
+    def upload_state(self, x4=None, v4=None):
+        n = self.particle_count
+        xa = None if x4 is None else np.ascontiguousarray(x4, np.float32).reshape(n, 4)
+        va = None if v4 is None else np.ascontiguousarray(v4, np.float32).reshape(n, 4)
+        C.check(C.lib().mtlp_upload_state(self._h, None if xa is None else _ptr(xa),
+                                          None if va is None else _ptr(va), n), "mtlp_upload_state")
+
+    # ---- Engine.hpp:78-90 ---------------------------------------------------
+    @property
+    def particle_count(self):
+        return C.lib().mtlp_particle_count(self._h)
+
+    @property
+    def total_cells(self):
+        return C.lib().mtlp_total_cells(self._h)
+
+    @property
+    def cell_size(self):
+        return C.lib().mtlp_cell_size(self._h)
+
+    def buffer(self, which):
+        return DeviceBuffer(self, which)
+
+    def positions_buffer(self):
+        return self.buffer(C.BUF_POSITIONS)
+
+    def velocities_buffer(self):
+        return self.buffer(C.BUF_VELOCITIES)
+
+    def prev_positions_buffer(self):
+        return self.buffer(C.BUF_PREV_POSITIONS)
+
+    def cell_hashes_buffer(self):
+        return self.buffer(C.BUF_CELL_HASHES)
+
+    def cell_start_buffer(self):
+        return self.buffer(C.BUF_CELL_START)
+
+    def sorted_index_buffer(self):
+        return self.buffer(C.BUF_SORTED_INDEX)
+
+    def neighbor_counts_buffer(self):
+        return self.buffer(C.BUF_NEIGHBOR_COUNTS)
+
+    def lambdas_buffer(self):
+        return self.buffer(C.BUF_LAMBDAS)
+
+    def sorted_positions_buffer(self):
+        return self.buffer(C.BUF_SORTED_POSITIONS)
+
+    # ---- diagnostics ------------------------------------------------------
+    def set_profiling(self, on=True):
+        C.check(C.lib().mtlp_set_profiling(self._h, 1 if on else 0), "mtlp_set_profiling")
+
+    def stage_times(self):
+        """[(kernel name, ms)] of the last step/build when profiling is on."""
+        cap = 1024
+        ms = (ctypes.c_float * cap)()
+        names = (ctypes.c_char_p * cap)()
+        count = ctypes.c_uint32()
+        C.check(C.lib().mtlp_stage_times(self._h, ms, names, cap, ctypes.byref(count)),
+                "mtlp_stage_times")
+        return [(names[i].decode(), ms[i]) for i in range(count.value)]
+
+    def error_flags(self):
+        f = ctypes.c_uint32()
+        C.check(C.lib().mtlp_error_flags(self._h, ctypes.byref(f)), "mtlp_error_flags")
+        return f.value

@@ -1,0 +1,270 @@
+"""numpy front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product (mtlphys_amd) never
+imports it.  See pbf_oracle.h for what the oracle restates and what pins it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("gravity", ctypes.c_float * 3),
+        ("bounds_min", ctypes.c_float * 3),
+        ("bounds_max", ctypes.c_float * 3),
+        ("h", ctypes.c_float),
+        ("cell_size", ctypes.c_float),
+        ("rest_density", ctypes.c_float),
+        ("epsilon", ctypes.c_float),
+        ("scorr_k", ctypes.c_float),
+        ("scorr_dq", ctypes.c_float),
+        ("max_step", ctypes.c_float),
+        ("damping", ctypes.c_float),
+        ("wall_eps", ctypes.c_float),
+        ("floor_friction", ctypes.c_float),
+        ("wall_friction", ctypes.c_float),
+        ("vmax", ctypes.c_float),
+        ("scorr_n", ctypes.c_uint32),
+        ("iterations", ctypes.c_uint32),
+    ]
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [
+        ("origin", ctypes.c_float * 3),
+        ("cell_size", ctypes.c_float),
+        ("inv_cell", ctypes.c_float),
+        ("dim", ctypes.c_uint32 * 3),
+        ("total_cells", ctypes.c_uint32),
+    ]
+
+
+def _trace_struct(real):
+    P = ctypes.POINTER
+    return type("Trace", (ctypes.Structure,), {"_fields_": [
+        ("keys", P(ctypes.c_uint32)), ("cell_start", P(ctypes.c_uint32)),
+        ("sorted_index", P(ctypes.c_uint32)), ("nbr_count", P(ctypes.c_uint32)),
+        ("xstar4", P(ctypes.c_float)), ("lambda_iter", P(real)), ("sx_iter", P(real)),
+        ("lambda_orig", P(real))]})
+
+
+TraceF64 = _trace_struct(ctypes.c_double)
+TraceF32 = _trace_struct(ctypes.c_float)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run `python oracle/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        u32 = ctypes.c_uint32
+        PP = ctypes.POINTER(OrcParams)
+        PG = ctypes.POINTER(OrcGrid)
+        sig = {
+            "orc_default_params": (None, [PP]),
+            "orc_grid_make": (None, [PP, PG]),
+            "orc_predict_f32": (None, [vp, vp, u32, vp, ctypes.c_float, vp]),
+            "orc_cell_key": (u32, [vp, PG]),
+            "orc_hash": (None, [vp, u32, PG, vp]),
+            "orc_count": (None, [vp, u32, u32, vp]),
+            "orc_exclusive_scan": (None, [vp, vp, u32]),
+            "orc_stable_scatter": (None, [vp, u32, vp, vp]),
+            "orc_build_spatial_hash": (None, [vp, u32, PG, vp, vp, vp, vp, ctypes.c_float, vp]),
+            "orc_density_lambda_f64": (None, [vp, vp, vp, u32, PG, PP, vp, vp, vp]),
+            "orc_density_lambda_f32": (None, [vp, vp, vp, u32, PG, PP, vp, vp, vp]),
+            "orc_apply_delta_f64": (None, [vp, vp, vp, u32, PG, PP, vp]),
+            "orc_apply_delta_f32": (None, [vp, vp, vp, u32, PG, PP, vp]),
+            "orc_finalize_f64": (None, [vp, vp, vp, u32, PP, ctypes.c_float]),
+            "orc_finalize_f32": (None, [vp, vp, vp, u32, PP, ctypes.c_float]),
+            "orc_step_f64": (ctypes.c_int, [vp, vp, u32, PP, ctypes.c_float, vp, vp,
+                                            ctypes.POINTER(TraceF64)]),
+            "orc_step_f32": (ctypes.c_int, [vp, vp, u32, PP, ctypes.c_float, vp, vp,
+                                            ctypes.POINTER(TraceF32)]),
+            "orc_set_threads": (None, [ctypes.c_int]),
+            "orc_get_threads": (ctypes.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def params(src=None, **overrides):
+    """OrcParams from the reference defaults, a ctypes struct with the same
+    field names (e.g. mtlphys_amd.Params), or a dict; then overrides."""
+    p = OrcParams()
+    lib().orc_default_params(ctypes.byref(p))
+    items = {}
+    if src is not None:
+        if isinstance(src, dict):
+            items.update(src)
+        else:
+            items.update({name: getattr(src, name) for name, _ in OrcParams._fields_})
+    items.update(overrides)
+    for k, v in items.items():
+        if hasattr(v, "__len__"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def grid(p):
+    g = OrcGrid()
+    lib().orc_grid_make(ctypes.byref(p), ctypes.byref(g))
+    return g
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def f32x4(a):
+    a = np.ascontiguousarray(a, np.float32)
+    assert a.ndim == 2 and a.shape[1] == 4
+    return a
+
+
+def predict(x4, v4, p, dt):
+    x4, v4 = f32x4(x4), f32x4(v4)
+    out = np.empty_like(x4)
+    g = (ctypes.c_float * 3)(*p.gravity)
+    lib().orc_predict_f32(_p(x4), _p(v4), len(x4), g, ctypes.c_float(dt), _p(out))
+    return out
+
+
+def cell_key(pt, g):
+    a = np.ascontiguousarray(pt, np.float32)
+    return lib().orc_cell_key(_p(a), ctypes.byref(g))
+
+
+def hash_keys(p4, g):
+    p4 = f32x4(p4)
+    out = np.empty(len(p4), np.uint32)
+    lib().orc_hash(_p(p4), len(p4), ctypes.byref(g), _p(out))
+    return out
+
+
+def count(keys, total_cells):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    out = np.empty(total_cells + 1, np.uint32)
+    lib().orc_count(_p(keys), len(keys), total_cells, _p(out))
+    return out
+
+
+def exclusive_scan(a):
+    a = np.ascontiguousarray(a, np.uint32)
+    out = np.empty_like(a)
+    lib().orc_exclusive_scan(_p(a), _p(out), len(a))
+    return out
+
+
+def stable_scatter(keys, cell_start):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    cs = np.ascontiguousarray(cell_start, np.uint32)
+    out = np.empty(len(keys), np.uint32)
+    lib().orc_stable_scatter(_p(keys), len(keys), _p(cs), _p(out))
+    return out
+
+
+def build_spatial_hash(p4, g, r2max):
+    """-> dict(keys, cell_start, sorted_index, sorted4, nbr)"""
+    p4 = f32x4(p4)
+    n = len(p4)
+    keys = np.empty(n, np.uint32)
+    cs = np.empty(g.total_cells + 1, np.uint32)
+    sidx = np.empty(n, np.uint32)
+    s4 = np.empty((n, 4), np.float32)
+    nbr = np.empty(n, np.uint32)
+    lib().orc_build_spatial_hash(_p(p4), n, ctypes.byref(g), _p(keys), _p(cs), _p(sidx), _p(s4),
+                                 ctypes.c_float(r2max), _p(nbr))
+    return dict(keys=keys, cell_start=cs, sorted_index=sidx, sorted4=s4, nbr=nbr)
+
+
+def _real(prec):
+    return (np.float64, "f64") if prec == 64 else (np.float32, "f32")
+
+
+def density_lambda(sx4, cell_start, sorted_index, g, p, prec=64):
+    dt, sfx = _real(prec)
+    sx4 = np.ascontiguousarray(sx4, dt)
+    n = len(sx4)
+    lam_s = np.empty(n, dt)
+    lam_o = np.empty(n, dt)
+    nbr = np.empty(n, np.uint32)
+    getattr(lib(), "orc_density_lambda_" + sfx)(
+        _p(sx4), _p(np.ascontiguousarray(cell_start, np.uint32)),
+        _p(np.ascontiguousarray(sorted_index, np.uint32)), n, ctypes.byref(g), ctypes.byref(p),
+        _p(lam_s), _p(lam_o), _p(nbr))
+    return lam_s, lam_o, nbr
+
+
+def apply_delta(sx4, lam_sorted, cell_start, g, p, prec=64):
+    dt, sfx = _real(prec)
+    sx4 = np.ascontiguousarray(sx4, dt)
+    out = np.empty_like(sx4)
+    getattr(lib(), "orc_apply_delta_" + sfx)(
+        _p(sx4), _p(np.ascontiguousarray(lam_sorted, dt)),
+        _p(np.ascontiguousarray(cell_start, np.uint32)), len(sx4), ctypes.byref(g),
+        ctypes.byref(p), _p(out))
+    return out
+
+
+def finalize(x4, v4, prev4, p, dt, prec=64):
+    rdt, sfx = _real(prec)
+    x = np.array(x4, rdt, copy=True, order="C")
+    v = np.array(v4, rdt, copy=True, order="C")
+    getattr(lib(), "orc_finalize_" + sfx)(_p(x), _p(v), _p(f32x4(prev4)), len(x), ctypes.byref(p),
+                                          ctypes.c_float(dt))
+    return x, v
+
+
+def step(x4, v4, p, dt, prec=64, trace=False):
+    """Engine::step on the oracle.  Returns (x4, v4[, trace dict])."""
+    rdt, sfx = _real(prec)
+    x4, v4 = f32x4(x4), f32x4(v4)
+    n = len(x4)
+    xo = np.empty((n, 4), rdt)
+    vo = np.empty((n, 4), rdt)
+    tr = None
+    arrays = None
+    if trace:
+        g = grid(p)
+        it = p.iterations
+        arrays = dict(keys=np.empty(n, np.uint32), cell_start=np.empty(g.total_cells + 1, np.uint32),
+                      sorted_index=np.empty(n, np.uint32), nbr_count=np.empty(n, np.uint32),
+                      xstar4=np.empty((n, 4), np.float32), lambda_iter=np.empty((it, n), rdt),
+                      sx_iter=np.empty((it + 1, n, 4), rdt), lambda_orig=np.empty(n, rdt))
+        T = TraceF64 if prec == 64 else TraceF32
+        kw = {}
+        for name, arr in arrays.items():
+            ctype = dict(T._fields_)[name]._type_
+            kw[name] = arr.ctypes.data_as(ctypes.POINTER(ctype))
+        tr = T(**kw)
+    getattr(lib(), "orc_step_" + sfx)(_p(x4), _p(v4), n, ctypes.byref(p), ctypes.c_float(dt),
+                                      _p(xo), _p(vo), ctypes.byref(tr) if tr is not None else None)
+    if trace:
+        return xo, vo, arrays
+    return xo, vo

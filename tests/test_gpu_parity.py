@@ -1,0 +1,416 @@
+"""GPU parity: the HIP kernels of libmtlp.so against the CPU oracle.
+
+Integer stages (cell keys, counts/offsets, permutation, neighbour counts) are
+compared bit for bit.  Floating-point stages are compared kernel by kernel:
+each GPU kernel's input is read back and fed to the FP64 oracle kernel, and
+the outputs must agree within
+    lambda: |dl| <= 1e-4 * max(|lambda_ref|, 1e-3 max|lambda_ref|)
+    x:      |dx| <= 1e-5 * h   (every component)
+    v:      |dv| <= 1e-5 * h / dt
+Free-running steps (GPU from the same (x, v) as the FP64 oracle) are
+reported, and bounded by the error of the oracle's own FP32 instantiation.
+"""
+import numpy as np
+import pytest
+
+from helpers import lambda_errors, position_error_h, run_gpu_step
+
+pytestmark = pytest.mark.gpu
+
+LAMBDA_TOL = 1e-4
+X_TOL_H = 1e-5
+
+
+# ---------------------------------------------------------------- scan ----
+def _gpu_scan(mtlp, arr):
+    import torch
+    n = len(arr)
+    d_in = torch.from_numpy(arr.astype(np.uint32).view(np.int32)).cuda()
+    d_out = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+    scratch = torch.empty(mtlp.Engine.scan_scratch_bytes(n), dtype=torch.uint8, device="cuda")
+    st = torch.cuda.current_stream()
+    mtlp.capi.check(mtlp.capi.lib().mtlp_exclusive_scan(
+        None, st.cuda_stream, d_in.data_ptr(), d_out.data_ptr(), scratch.data_ptr(), n),
+        "mtlp_exclusive_scan")
+    torch.cuda.synchronize()
+    return d_out.cpu().numpy().view(np.uint32)
+
+
+def _ref_scan(a):
+    out = np.zeros(len(a), np.uint64)
+    out[1:] = np.cumsum(a[:-1].astype(np.uint64))
+    return (out & 0xFFFFFFFF).astype(np.uint32)
+
+
+# The reference's scan cases (tests/test_scan.cpp:59-110), inputs from this
+# project's RNG (std::uniform_int_distribution is implementation-defined),
+# plus tile edges and lengths past the reference's 1024^2 cap.
+SCAN_CASES = [("zeros", 100, 0, 0), ("ones", 500, 1, 1), ("rand", 1024, 0xA, 7), ("rand", 700, 0xB, 7),
+              ("rand", 2048, 0xC, 7), ("rand", 635626, 0xD, 3), ("rand", 1, 1, 7), ("rand", 4095, 2, 7),
+              ("rand", 4096, 3, 7), ("rand", 4097, 4, 7), ("rand", 530605, 5, 7),
+              ("rand", (1 << 20) + 1, 6, 7), ("rand", 2097153, 7, 7), ("rand", (1 << 24) + 1, 8, 7),
+              ("big", 100000, 9, 0xFFFFFFFF)]
+
+
+@pytest.mark.parametrize("kind,n,seed,maxv", SCAN_CASES)
+def test_exclusive_scan(mtlp, kind, n, seed, maxv):
+    rng = np.random.default_rng(seed)
+    if kind == "zeros":
+        a = np.zeros(n, np.uint32)
+    elif kind == "ones":
+        a = np.ones(n, np.uint32)
+    else:
+        a = rng.integers(0, maxv, n, dtype=np.uint64, endpoint=True).astype(np.uint32)
+    got = _gpu_scan(mtlp, a)
+    np.testing.assert_array_equal(got, _ref_scan(a))
+
+
+def test_scan_repeated_launches_epochs(mtlp):
+    """The engine's scan reuses its look-back words across launches."""
+    p = mtlp.scene_params(3, 32768)
+    x, v = mtlp.generate_scene(3, 32768, 1, p)
+    with mtlp.Engine(0, p) as e:
+        e.reset(32768, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(x, v)
+        first = None
+        for _ in range(5):
+            e.build_spatial_hash()
+            cs = e.cell_start_buffer().read()
+            if first is None:
+                first = cs
+            np.testing.assert_array_equal(cs, first)
+        assert e.error_flags() == 0
+
+
+# ------------------------------------------------------- spatial hash ----
+def _hash_case(mtlp, orc, params, pts):
+    pts = np.ascontiguousarray(pts, np.float32)
+    n = len(pts)
+    with mtlp.Engine(0, params) as e:
+        e.reset(n, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(pts, np.zeros_like(pts))
+        e.build_spatial_hash()
+        got = dict(keys=e.cell_hashes_buffer().read(), cell_start=e.cell_start_buffer().read(),
+                   sorted_index=e.sorted_index_buffer().read(),
+                   sorted4=e.sorted_positions_buffer().read(), nbr=e.neighbor_counts_buffer().read())
+        assert e.error_flags() == 0
+        assert e.total_cells == int(np.prod(mtlp.grid_dims(params)))
+    op = orc.params(params)
+    ref = orc.build_spatial_hash(pts, orc.grid(op), np.float32(op.h) * np.float32(op.h))
+    return got, ref
+
+
+def _assert_hash_equal(got, ref):
+    for k in ("keys", "cell_start", "sorted_index", "nbr"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    np.testing.assert_array_equal(got["sorted4"].view(np.uint32), ref["sorted4"].view(np.uint32))
+
+
+def test_hash_reference_known_answers(mtlp, orc):
+    """tests/test_spatial.cpp:52-70 (reference grid: [-3,3]x[-3,6]x[-3,3], cell 0.1)."""
+    pts = np.array([[0.0, 0.0, 0.0, 1], [2.5, 4.0, -1.5, 1], [-2.9, 5.5, 2.5, 1], [1.0, -2.0, -2.5, 1]],
+                   np.float32)
+    got, ref = _hash_case(mtlp, orc, mtlp.default_params(), pts)
+    # independent FP32 restatement of cpuCellHash (tests/TestUtils.cpp:30-42)
+    o = np.float32(-3.0)
+    inv = np.float32(1.0) / np.float32(0.1)
+    dims = np.array([60, 90, 60])
+    c = np.floor((pts[:, :3] - o) * inv).astype(np.int64)
+    c = np.clip(c, 0, dims - 1)
+    expect = c[:, 0] + c[:, 1] * 60 + c[:, 2] * 60 * 90
+    np.testing.assert_array_equal(got["keys"], expect)
+    _assert_hash_equal(got, ref)
+
+
+def test_hash_grouping_reference_case(mtlp, orc):
+    """tests/test_spatial.cpp:72-100: cell ranges hold exactly their particles,
+    here in increasing index order."""
+    pts = np.array([[0, 0, 0, 1], [0.01, 0.02, 0.01, 1], [0.03, 0.01, 0.02, 1], [2, 2, 2, 1],
+                    [2.01, 2.02, 2.03, 1], [-2.5, -2.5, -2.5, 1]], np.float32)
+    got, ref = _hash_case(mtlp, orc, mtlp.default_params(), pts)
+    _assert_hash_equal(got, ref)
+    assert got["cell_start"][-1] == len(pts)
+    for key in np.unique(got["keys"]):
+        s, e = got["cell_start"][key], got["cell_start"][key + 1]
+        np.testing.assert_array_equal(got["sorted_index"][s:e], np.nonzero(got["keys"] == key)[0])
+
+
+@pytest.mark.parametrize("case", ["isolated", "pair", "lattice"])
+def test_neighbor_counts_reference_cases(mtlp, orc, case):
+    """tests/test_spatial.cpp:102-145."""
+    cs = 0.1
+    if case == "isolated":
+        pts = [[0, 0, 0, 1], [1.5, 1.5, 1.5, 1], [-1.5, -1.5, -1.5, 1]]
+        expect = {0: 0, 1: 0, 2: 0}
+    elif case == "pair":
+        pts = [[0, 0, 0, 1], [cs * 0.5, 0, 0, 1]]
+        expect = {0: 1, 1: 1}
+    else:
+        s = np.float32(cs) * np.float32(0.99)
+        pts = [[x * s, y * s, z * s, 1] for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)]
+        expect = {13: 6}
+    got, ref = _hash_case(mtlp, orc, mtlp.default_params(), np.array(pts, np.float32))
+    _assert_hash_equal(got, ref)
+    for i, cnt in expect.items():
+        assert got["nbr"][i] == cnt
+    if case == "lattice":
+        assert got["nbr"][0] < got["nbr"][13] and got["nbr"][26] < got["nbr"][13]
+
+
+def _unit_grid_params(mtlp, cells=128):
+    p = mtlp.default_params()
+    for a in range(3):
+        p.bounds_min[a] = 0.0
+        p.bounds_max[a] = 1.0
+    p.cell_size = 1.0 / cells
+    p.h = 1.0 / cells
+    return p
+
+
+def _clustered_points(n, seed, cells=128, frac_cells=0.05, frac_pts=0.9):
+    rng = np.random.default_rng(seed)
+    total = cells ** 3
+    hot = rng.permutation(total)[: int(round(frac_cells * total))]
+    n_hot = int(round(frac_pts * n))
+    pick = hot[rng.integers(0, len(hot), n_hot)]
+    cx, cy, cz = pick % cells, (pick // cells) % cells, pick // (cells * cells)
+    off = rng.random((n_hot, 3), dtype=np.float32)
+    hot_pts = (np.stack([cx, cy, cz], 1) + off * np.float32(0.999)) / np.float32(cells)
+    cold_pts = rng.random((n - n_hot, 3), dtype=np.float32)
+    pts = np.concatenate([hot_pts.astype(np.float32), cold_pts])[rng.permutation(n)]
+    return np.concatenate([pts, np.ones((n, 1), np.float32)], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("variant", ["uniform", "clustered"])
+def test_hash_4m_128cubed_bitexact(mtlp, orc, variant):
+    """4M points in a 128^3 grid (2,097,152 cells; scan length 2,097,153)."""
+    n = 4 * 1024 * 1024
+    if variant == "uniform":
+        rng = np.random.default_rng(2)
+        pts = np.concatenate([rng.random((n, 3), dtype=np.float32), np.ones((n, 1), np.float32)], 1)
+    else:
+        pts = _clustered_points(n, 2)
+    p = _unit_grid_params(mtlp)
+    with mtlp.Engine(0, p) as e:
+        e.reset(n, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(pts, np.zeros_like(pts))
+        e.build_spatial_hash()
+        keys = e.cell_hashes_buffer().read()
+        cs = e.cell_start_buffer().read()
+        sidx = e.sorted_index_buffer().read()
+        s4 = e.sorted_positions_buffer().read()
+        assert e.error_flags() == 0
+    op = orc.params(p)
+    g = orc.grid(op)
+    ref_keys = orc.hash_keys(pts, g)
+    np.testing.assert_array_equal(keys, ref_keys)
+    counts = np.bincount(ref_keys, minlength=g.total_cells + 1).astype(np.uint32)
+    np.testing.assert_array_equal(cs, _ref_scan(counts))
+    np.testing.assert_array_equal(sidx, np.argsort(ref_keys, kind="stable").astype(np.uint32))
+    np.testing.assert_array_equal(s4.view(np.uint32), pts[sidx].view(np.uint32))
+
+
+@pytest.mark.parametrize("n_in_cell", [300, 5000, 16384, 16385, 70000])
+def test_hash_big_cells_stable(mtlp, orc, n_in_cell):
+    """Cells above the in-place ranking limit: LDS sort up to 16384 particles,
+    ordered key scan beyond; mixed with ordinary cells."""
+    rng = np.random.default_rng(n_in_cell)
+    n_rest = 20000
+    hot = np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (n_in_cell, 1))
+    hot += rng.random((n_in_cell, 3), dtype=np.float32) * np.float32(0.5 / 128)
+    rest = rng.random((n_rest, 3), dtype=np.float32)
+    pts = np.concatenate([hot, rest])[rng.permutation(n_in_cell + n_rest)]
+    pts = np.concatenate([pts, np.ones((len(pts), 1), np.float32)], 1).astype(np.float32)
+    p = _unit_grid_params(mtlp)
+    with mtlp.Engine(0, p) as e:
+        e.reset(len(pts), mtlp.capi.SCENE_EMPTY)
+        e.upload_state(pts, np.zeros_like(pts))
+        e.build_spatial_hash()
+        keys = e.cell_hashes_buffer().read()
+        sidx = e.sorted_index_buffer().read()
+        s4 = e.sorted_positions_buffer().read()
+        assert e.error_flags() == 0
+    np.testing.assert_array_equal(sidx, np.argsort(keys, kind="stable").astype(np.uint32))
+    np.testing.assert_array_equal(s4.view(np.uint32), pts[sidx].view(np.uint32))
+
+
+# ---------------------------------------------------------------- step ----
+def _cube_case(mtlp, n=32768, seed=1):
+    p = mtlp.scene_params(3, n)
+    x, v = mtlp.generate_scene(3, n, seed, p)
+    return p, x, v
+
+
+def test_step_integer_stages_bitexact(mtlp, orc):
+    p, x, v = _cube_case(mtlp)
+    dt = 1.0 / 120.0
+    got = run_gpu_step(mtlp, p, x, v, dt)
+    op = orc.params(p)
+    _, _, tr = orc.step(x, v, op, dt, prec=32, trace=True)
+    for k in ("keys", "cell_start", "sorted_index"):
+        np.testing.assert_array_equal(got[k], tr[k], err_msg=k)
+    # iteration-0 neighbour count == countNeighbors on the predicted positions
+    np.testing.assert_array_equal(got["nbr"], tr["nbr_count"])
+    np.testing.assert_array_equal(got["prev"].view(np.uint32), x.view(np.uint32))
+
+
+def _kernel_level(mtlp, orc, p, x, v, dt):
+    """Run the GPU with 1..I iterations; check every lambda/delta kernel and
+    finalize against the FP64 oracle fed the GPU's own inputs."""
+    op = orc.params(p)
+    g = orc.grid(op)
+    h = op.h
+    report = []
+    prev_sorted = None
+    runs = [run_gpu_step(mtlp, p, x, v, dt, iterations=k) for k in range(0, p.iterations + 1)]
+    cs, sidx = runs[0]["cell_start"], runs[0]["sorted_index"]
+    xs = orc.predict(x, v, op, dt)
+    np.testing.assert_array_equal(runs[0]["sorted"].view(np.uint32)[:, :3],
+                                  xs[sidx].view(np.uint32)[:, :3])
+    for k in range(p.iterations):
+        inp = runs[k]["sorted"].astype(np.float64)
+        lam_s, lam_o, _ = orc.density_lambda(inp, cs, sidx, g, op, 64)
+        le = lambda_errors(runs[k + 1]["lam"], lam_o)
+        nxt = orc.apply_delta(inp, runs[k + 1]["lam"][sidx].astype(np.float64), cs, g, op, 64)
+        xe = position_error_h(runs[k + 1]["sorted"], nxt, h)
+        report.append((k, le, xe))
+        assert le["rel_floor"] <= LAMBDA_TOL, (k, le)
+        assert xe["max"] <= X_TOL_H, (k, xe)
+    final = runs[-1]
+    xo = np.empty((len(x), 4), np.float64)
+    xo[sidx] = final["sorted"].astype(np.float64)
+    xr, vr = orc.finalize(xo, v, x, op, dt, 64)
+    xe = position_error_h(final["x"], xr, h)
+    ve = np.abs(final["v"][:, :3] - vr[:, :3]).max() / (h / dt)
+    report.append(("finalize", xe, ve))
+    assert xe["max"] <= X_TOL_H and ve <= X_TOL_H, report[-1]
+    return report
+
+
+def test_step_kernel_level_cube(mtlp, orc):
+    p, x, v = _cube_case(mtlp)
+    _kernel_level(mtlp, orc, p, x, v, 1.0 / 120.0)
+
+
+def test_step_kernel_level_after_settling(mtlp, orc):
+    """Same check on a state 20 GPU steps into the cube drop."""
+    p, x, v = _cube_case(mtlp, n=8000, seed=3)
+    with mtlp.Engine(0, p) as e:
+        e.reset(len(x), mtlp.capi.SCENE_EMPTY)
+        e.upload_state(x, v)
+        for _ in range(20):
+            e.step(1.0 / 120.0)
+        x1, v1 = e.positions_buffer().read(), e.velocities_buffer().read()
+    assert np.isfinite(x1).all() and np.isfinite(v1).all()
+    _kernel_level(mtlp, orc, p, x1, v1, 1.0 / 120.0)
+
+
+def test_step_free_running_vs_fp64(mtlp, orc):
+    """One full step from the same state: the GPU may differ from the FP64
+    oracle by no more than a few times the oracle's own FP32 instantiation
+    (the Jacobi iterations amplify FP32 rounding in a few particles)."""
+    p, x, v = _cube_case(mtlp)
+    dt = 1.0 / 120.0
+    op = orc.params(p)
+    got = run_gpu_step(mtlp, p, x, v, dt)
+    x64, v64, tr = orc.step(x, v, op, dt, prec=64, trace=True)
+    x32, v32, t32 = orc.step(x, v, op, dt, prec=32, trace=True)
+    e_gpu = position_error_h(got["x"], x64, op.h)
+    e_f32 = position_error_h(x32, x64, op.h)
+    l_gpu = lambda_errors(got["lam"], tr["lambda_orig"])
+    l_f32 = lambda_errors(t32["lambda_orig"], tr["lambda_orig"])
+    print("free-running x err/h gpu", e_gpu, "oracle-f32", e_f32, "lambda gpu", l_gpu, "f32", l_f32)
+    assert e_gpu["p50"] <= X_TOL_H
+    assert e_gpu["max"] <= 4 * e_f32["max"] + X_TOL_H
+    assert l_gpu["rel_max"] <= max(LAMBDA_TOL, 4 * l_f32["rel_max"])
+
+
+def test_dam_break_1m_step(mtlp, orc):
+    """The 1M-particle dam break (h = 0.02, 530,604 cells, 4 iterations):
+    integer stages bit-exact, iteration-0 kernels and the full step checked
+    against the FP64 oracle."""
+    n = 1_000_000
+    p = mtlp.scene_params(4, n)
+    assert mtlp.grid_dims(p) == (102, 102, 51)
+    x, v = mtlp.generate_scene(4, n, 1, p)
+    dt = 1.0 / 120.0
+    op = orc.params(p)
+    got1 = run_gpu_step(mtlp, p, x, v, dt, iterations=1)
+    got0 = run_gpu_step(mtlp, p, x, v, dt, iterations=0)
+    x64, v64, tr = orc.step(x, v, orc.params(p, iterations=1), dt, prec=64, trace=True)
+    for k in ("keys", "cell_start", "sorted_index"):
+        np.testing.assert_array_equal(got1[k], tr[k], err_msg=k)
+    np.testing.assert_array_equal(got1["nbr"], tr["nbr_count"])
+    le = lambda_errors(got1["lam"], tr["lambda_orig"])
+    assert le["rel_floor"] <= LAMBDA_TOL, le
+    g = orc.grid(op)
+    nxt = orc.apply_delta(got0["sorted"].astype(np.float64),
+                          got1["lam"][got1["sorted_index"]].astype(np.float64),
+                          got1["cell_start"], g, op, 64)
+    xe = position_error_h(got1["sorted"], nxt, op.h)
+    assert xe["max"] <= X_TOL_H, xe
+
+
+# -------------------------------------------- reference pbf test cases ----
+def test_isolated_particle_free_fall(mtlp):
+    """tests/test_pbf.cpp:47-65: y = 1 + (-9.81 * 0.01) * 0.01 after dt = 0.01."""
+    with mtlp.Engine(0) as e:
+        e.reset(1)
+        pos = e.positions_buffer()
+        pos.write(np.array([[0, 1, 0, 1]], np.float32))
+        e.velocities_buffer().write(np.zeros((1, 4), np.float32))
+        e.step(0.01)
+        y = pos.read()[0, 1]
+        vy = e.velocities_buffer().read()[0, 1]
+    assert abs(y - 0.99902) <= 1e-3 * 0.99902
+    assert y == np.float32(1.0) + (np.float32(0.0) + np.float32(-9.81) * np.float32(0.01)) * np.float32(0.01)
+    assert vy < 0
+
+
+def test_lambda_sign_and_isolated_closed_form(mtlp):
+    """tests/test_pbf.cpp:67-107 with the engine's own epsilon (800)."""
+    h = 0.1
+    s = np.float32(h * 0.4)
+    pts = [[x * s, y * s, z * s, 1] for z in (-1, 0, 1) for y in (-1, 0, 1) for x in (-1, 0, 1)]
+    pts.append([2, 2, 2, 1])
+    pts = np.array(pts, np.float32)
+    p = mtlp.default_params()
+    with mtlp.Engine(0, p) as e:
+        e.reset(len(pts))
+        e.upload_state(pts, np.zeros_like(pts))
+        e.step(1e-6)
+        lam = e.lambdas_buffer().read()
+    assert lam[13] < 0
+    w0 = 315.0 / (64.0 * np.pi * h ** 3)
+    expect = -(w0 / 8000.0 - 1.0) / 800.0
+    assert abs(lam[-1] - expect) <= 1e-5 * abs(expect)
+
+
+def test_fifty_steps_stable(mtlp):
+    """tests/test_pbf.cpp:109-130: 1024 particles, 50 steps at dt = 1/240."""
+    with mtlp.Engine(0) as e:
+        e.reset(1024)
+        for _ in range(50):
+            e.step(1.0 / 240.0)
+        x = e.positions_buffer().read()
+    assert np.isfinite(x).all()
+    lo, hi = np.array([-3.01, -3.01, -3.01]), np.array([3.01, 6.01, 3.01])
+    assert ((x[:, :3] >= lo) & (x[:, :3] <= hi)).all()
+
+
+def test_dt_nonpositive_is_noop(mtlp):
+    with mtlp.Engine(0) as e:
+        e.reset(1000)
+        x0 = e.positions_buffer().read()
+        e.step(0.0)
+        e.step(-1.0)
+        np.testing.assert_array_equal(e.positions_buffer().read(), x0)
+
+
+def test_zero_particles(mtlp):
+    with mtlp.Engine(0) as e:
+        e.reset(0)
+        e.step(0.01)
+        e.build_spatial_hash()
+        assert e.particle_count == 0
+        assert e.cell_start_buffer().read()[-1] == 0
