@@ -113,6 +113,18 @@ class MtlpError(RuntimeError):
         super().__init__(f"{where}: {name}" + (f" ({message})" if message else ""))
 
 
+def _share_torch_hip_runtime():
+    """A process must hold one HIP runtime.  torch's wheel carries its own
+    libamdhip64.so (SONAME libamdhip64.so.7, like /opt/rocm's); loading torch
+    first makes the dynamic loader resolve libmtlp.so's libamdhip64.so.7 to
+    that same copy.  Loaded the other way round, the process would hold two
+    runtimes and the second finds no device."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     """Load libmtlp.so, failing loudly if it has not been built."""
     global _lib
@@ -120,6 +132,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with "
                                "`python -m mtlphys_amd.build` (or __graft_entry__.build())")
+        _share_torch_hip_runtime()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

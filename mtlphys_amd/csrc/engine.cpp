@@ -119,6 +119,12 @@ SolverDev make_solver(const mtlp_params& p) {
     s.poly6_d = 315.0 / (64.0 * 3.14159265358979323846 * h9d);
     s.w0_d = s.poly6_d * h2d * h2d * h2d;
     s.inv_rho0_d = 1.0 / (double)p.rest_density;
+    s.h_d = hd;
+    s.spiky_d = 45.0 / (3.14159265358979323846 * h6d);
+    const double dqhd = (double)p.scorr_dq * hd, ddd = h2d - dqhd * dqhd;
+    s.inv_wq_d = 1.0 / (s.poly6_d * ddd * ddd * ddd);
+    s.scorr_k_d = p.scorr_k;
+    s.max_step_d = (double)p.max_step * hd;
     s.bminx = p.bounds_min[0];
     s.bminy = p.bounds_min[1];
     s.bminz = p.bounds_min[2];

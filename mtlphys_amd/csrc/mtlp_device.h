@@ -61,4 +61,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own L2.
+// Renumber so that XCD x runs one contiguous run of logical blocks: slots
+// that share stencil rows then share an L2.  A bijection on [0, nb).
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t x = b % kNumXcd, i = b / kNumXcd;
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    return x * q + min(x, r) + i;
+}
+
 }  // namespace mtlp

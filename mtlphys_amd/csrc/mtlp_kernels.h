@@ -44,6 +44,10 @@ struct PredictDev {
 // ~1e-5 relative in lambda.
 struct SolverDev {
     double poly6_d, h2_d, w0_d, inv_rho0_d;
+    // The delta pass runs its per-neighbour terms in FP64 too: with a
+    // strong s_corr the terms cancel to a small sum that the 0.3 h cap then
+    // rescales, which turns FP32 term rounding into a direction error.
+    double h_d, spiky_d, inv_wq_d, scorr_k_d, max_step_d;
     float h, h2, poly6, spiky, inv_rho0, eps;
     float w0;             // poly6(0)
     float inv_wq;         // 1 / poly6((dq h)^2)

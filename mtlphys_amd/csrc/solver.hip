@@ -33,8 +33,80 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
     return rx * rx + ry * ry + rz * rz;
 }
 
-__device__ __forceinline__ float ipow(float x, uint32_t n) {
-    float result = 1.0f, b = x;
+// Candidates are filtered before any SPH arithmetic: bit k of the result is
+// set when slot c0 + k (k < c1 - c0 <= 32) is a neighbour of s, i.e.
+// |p_s - p_j|^2 < h^2 and j != s.  The expensive terms then run only on set
+// bits, so lanes of a wave stay busy on real neighbours.  Loads go out eight
+// at a time (addresses clamped into [c0, c1)) before any is consumed.
+__device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur, uint32_t c0,
+                                                  uint32_t c1, uint32_t s, float4 pi, float h2) {
+    uint32_t mask = 0;
+    const uint32_t last = c1 - 1;
+#pragma unroll
+    for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
+        if (c0 + k0 >= c1) break;
+        float4 q[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) q[u] = cur[min(c0 + k0 + u, last)];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t j = c0 + k0 + u;
+            const float r2 = dist2(pi.x - q[u].x, pi.y - q[u].y, pi.z - q[u].z);
+            mask |= (j < c1 && j != s && r2 < h2) ? (1u << (k0 + u)) : 0u;
+        }
+    }
+    return mask;
+}
+
+// Slot ranges of the 9 stencil rows (z-major, then y), fetched together.
+// Rows outside the grid are empty.  Row (y, z) holds cells x0..x1, whose
+// slots are contiguous in cell order.
+struct Rows {
+    uint32_t a[9], b[9];
+};
+
+__device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* __restrict__ cell_start,
+                                             const GridDev& g, uint32_t n) {
+    Rows r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int z = st.z0 + k / 3, y = st.y0 + k % 3;
+        const bool ok = z <= st.z1 && y <= st.y1;
+        const uint32_t row = ok ? (uint32_t)y * g.dx + (uint32_t)z * g.dxy : 0u;
+        const uint32_t a = cell_start[row + (uint32_t)st.x0];
+        const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
+        r.a[k] = ok ? a : 0u;
+        r.b[k] = ok ? b : 0u;
+    }
+    return r;
+}
+
+// One neighbour's contribution to the density sum (FP64) and to the
+// constraint gradient terms (Constraints.metal:110-119).
+__device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
+                                             double& wsum, float& gx, float& gy, float& gz,
+                                             float& gsq) {
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float r2 = dist2(rx, ry, rz);
+    const double dxd = rx, dyd = ry, dzd = rz;
+    const double diff = sd.h2_d - (dxd * dxd + dyd * dyd + dzd * dzd);
+    wsum += diff * diff * diff;
+    // gradient terms only enter lambda's denominator: hardware 1/sqrt (~1 ulp)
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float rl = r2 * inv_r;
+    if (rl > 1e-6f && rl < sd.h) {
+        const float dd = sd.h - rl;
+        const float sc = -sd.spiky * dd * dd * inv_r;
+        const float ax = sc * rx, ay = sc * ry, az = sc * rz;
+        gx += ax;
+        gy += ay;
+        gz += az;
+        gsq += ax * ax + ay * ay + az * az;
+    }
+}
+
+__device__ __forceinline__ double ipow_d(double x, uint32_t n) {
+    double result = 1.0, b = x;
     while (n) {
         if (n & 1u) result *= b;
         n >>= 1;
@@ -43,12 +115,39 @@ __device__ __forceinline__ float ipow(float x, uint32_t n) {
     return result;
 }
 
+// One neighbour's term of the position correction (Constraints.metal:171-181),
+// in FP64 from the FP32 positions and lambdas (all exact in FP64).
+__device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const float4& pj,
+                                           const SolverDev& sd, double& dx, double& dy, double& dz) {
+    const double rx = (double)pi.x - (double)pj.x;
+    const double ry = (double)pi.y - (double)pj.y;
+    const double rz = (double)pi.z - (double)pj.z;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    // 1/r from the hardware estimate (~2^-23) plus one Newton step (~1e-14);
+    // r2 = 0 gives NaN here and fails the range test below, like r <= 1e-6.
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double inv_r = y0 * (1.5 - 0.5 * r2 * y0 * y0);
+    const double rl = r2 * inv_r;
+    double sc = 0.0;
+    if (rl > (double)1e-6f && rl < sd.h_d) {
+        const double dd = sd.h_d - rl;
+        sc = -sd.spiky_d * dd * dd * inv_r;
+    }
+    const double diff = sd.h2_d - r2;
+    const double wij = sd.poly6_d * diff * diff * diff;
+    const double scorr = -sd.scorr_k_d * ipow_d(wij * sd.inv_wq_d, sd.scorr_n);
+    const double coef = (lam_i + (double)pj.w + scorr) * sc;
+    dx += coef * rx;
+    dy += coef * ry;
+    dz += coef * rz;
+}
+
 template <bool kOrig, bool kCount>
 __global__ __launch_bounds__(kBlock) void k_density_lambda(
     float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, uint32_t n, GridDev g, SolverDev sd,
     float* __restrict__ lambda_orig, uint32_t* __restrict__ nbr_orig) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= n) return;
     const float4 pi = cur[s];
     const Stencil st = stencil_of(pi.x, pi.y, pi.z, g);
@@ -58,33 +157,26 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
     double wsum = 0.0;
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
-    for (int z = st.z0; z <= st.z1; ++z)
-        for (int y = st.y0; y <= st.y1; ++y) {
-            const uint32_t row = (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
-            const uint32_t a = cell_start[row + (uint32_t)st.x0];
-            const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
-            for (uint32_t j = a; j < b; ++j) {
-                const float4 pj = cur[j];
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float r2 = dist2(rx, ry, rz);
-                if (r2 < sd.h2 && j != s) {
-                    ++cnt;
-                    const double dxd = rx, dyd = ry, dzd = rz;
-                    const double diff = sd.h2_d - (dxd * dxd + dyd * dyd + dzd * dzd);
-                    wsum += diff * diff * diff;
-                    const float rl = sqrtf(r2);
-                    if (rl > 1e-6f && rl < sd.h) {
-                        const float dd = sd.h - rl;
-                        const float sc = -sd.spiky * dd * dd / rl;
-                        const float ax = sc * rx, ay = sc * ry, az = sc * rz;
-                        gx += ax;
-                        gy += ay;
-                        gz += az;
-                        gsq += ax * ax + ay * ay + az * az;
-                    }
-                }
+    const Rows rows = stencil_rows(st, cell_start, g, n);
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k) {
+        const uint32_t b = rows.b[k];
+        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
+            uint32_t mask = neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sd.h2);
+            cnt += (uint32_t)__popc(mask);
+            while (mask) {
+                // two neighbours per trip: both loads are in flight together
+                const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const bool two = mask != 0u;
+                const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
+                if (two) mask &= mask - 1u;
+                const float4 q1 = cur[j1], q2 = cur[j2];
+                density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
+                if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
             }
         }
+    }
     const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
     const float denom = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq) + sd.eps;
     const float lambda = (float)(-constraint / (double)denom);
@@ -99,51 +191,45 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
 __global__ __launch_bounds__(kBlock) void k_apply_delta(
     const float4* __restrict__ cur, const uint32_t* __restrict__ cell_start, uint32_t n, GridDev g,
     SolverDev sd, float4* __restrict__ next) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (s >= n) return;
     const float4 pi = cur[s];
     const float lam_i = pi.w;
     const Stencil st = stencil_of(pi.x, pi.y, pi.z, g);
-    float dx = 0.f, dy = 0.f, dz = 0.f;
-    for (int z = st.z0; z <= st.z1; ++z)
-        for (int y = st.y0; y <= st.y1; ++y) {
-            const uint32_t row = (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
-            const uint32_t a = cell_start[row + (uint32_t)st.x0];
-            const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
-            for (uint32_t j = a; j < b; ++j) {
-                const float4 pj = cur[j];
-                const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                const float r2 = dist2(rx, ry, rz);
-                if (r2 < sd.h2 && j != s) {
-                    const float rl = sqrtf(r2);
-                    float sc = 0.f;
-                    if (rl > 1e-6f && rl < sd.h) {
-                        const float dd = sd.h - rl;
-                        sc = -sd.spiky * dd * dd / rl;
-                    }
-                    const float diff = sd.h2 - r2;
-                    const float wij = sd.poly6 * diff * diff * diff;
-                    const float scorr = -sd.scorr_k * ipow(wij * sd.inv_wq, sd.scorr_n);
-                    const float coef = (lam_i + pj.w + scorr) * sc;
-                    dx += coef * rx;
-                    dy += coef * ry;
-                    dz += coef * rz;
-                }
+    const double lam_d = lam_i;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    const Rows rows = stencil_rows(st, cell_start, g, n);
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k) {
+        const uint32_t b = rows.b[k];
+        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
+            uint32_t mask = neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sd.h2);
+            while (mask) {
+                const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
+                mask &= mask - 1u;
+                const bool two = mask != 0u;
+                const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
+                if (two) mask &= mask - 1u;
+                const float4 q1 = cur[j1], q2 = cur[j2];
+                delta_term(pi, lam_d, q1, sd, dx, dy, dz);
+                if (two) delta_term(pi, lam_d, q2, sd, dx, dy, dz);
             }
         }
-    dx *= sd.inv_rho0;
-    dy *= sd.inv_rho0;
-    dz *= sd.inv_rho0;
-    const float dlen = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (dlen > sd.max_step) {
-        const float f = sd.max_step / dlen;
+    }
+    dx *= sd.inv_rho0_d;
+    dy *= sd.inv_rho0_d;
+    dz *= sd.inv_rho0_d;
+    const double dlen = sqrt(dx * dx + dy * dy + dz * dz);
+    if (dlen > sd.max_step_d) {
+        const double f = sd.max_step_d / dlen;
         dx *= f;
         dy *= f;
         dz *= f;
     }
-    next[s] = make_float4(fminf(fmaxf(pi.x + dx, sd.bminx), sd.bmaxx),
-                          fminf(fmaxf(pi.y + dy, sd.bminy), sd.bmaxy),
-                          fminf(fmaxf(pi.z + dz, sd.bminz), sd.bmaxz), 1.0f);
+    // clamp to the box (Constraints.metal:197), one rounding to FP32
+    next[s] = make_float4((float)fmin(fmax((double)pi.x + dx, (double)sd.bminx), (double)sd.bmaxx),
+                          (float)fmin(fmax((double)pi.y + dy, (double)sd.bminy), (double)sd.bmaxy),
+                          (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
 }
 
 // Scatter back + finalize: slot s writes particle sorted_index[s].  The old
