@@ -96,12 +96,60 @@ def cpu_baseline(x, v, params, iters_budget_s=20.0):
                       f"median; OpenMP over particles, serial scan/scatter"}
 
 
+def _time_steps(eng, steps, stream, torch):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.step(DT, stream)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def batched_leg(steps, warmup, stream, torch):
+    """256 independent cube drops (N_s log-uniform in [2048, 65536], per-scene
+    rest density and gravity, 3 iterations) in one batch, against one cube
+    drop of the same total particle count stepped alone."""
+    from mtlphys_amd import workloads as W
+    import mtlphys_amd as mtlp
+    b = W.make_ensemble()
+    n = b.particle_count
+    block = b.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"))
+    b.set_graph(True)
+    for _ in range(warmup):
+        b.step(DT, stream)
+    t_batch = _time_steps(b, steps, stream, torch)
+    _, co = b.offsets()
+    iters = b.scene_params[0].iterations
+    bytes_batch = sum(W.step_bytes(c, int(co[s + 1] - co[s]), iters) for s, c in enumerate(b.counts))
+    flags = b.error_flags()
+    b.close()
+    p, x, v = W.cube_drop(n)
+    single = mtlp.Engine(0, p)
+    single.reset(n, mtlp.capi.SCENE_EMPTY)
+    single.upload_state(x, v)
+    single.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"))
+    single.set_graph(True)
+    for _ in range(warmup):
+        single.step(DT, stream)
+    t_single = _time_steps(single, steps, stream, torch)
+    bytes_single = W.step_bytes(n, single.total_cells, p.iterations)
+    single.close()
+    gbs_b, gbs_s = bytes_batch / t_batch / 1e9, bytes_single / t_single / 1e9
+    return {"scenes": 256, "particles": n, "iterations": iters, "stencil_block": block,
+            "ms_per_step": round(t_batch * 1e3, 4), "value": round(gbs_b, 2), "unit": "GB/s",
+            "single_scene_ms_per_step": round(t_single * 1e3, 4), "single_scene_gbs": round(gbs_s, 2),
+            "ratio_to_single_scene": round(gbs_b / gbs_s, 3), "error_flags": flags,
+            "note": "one launch per stage for all scenes, HIP-graph replay; the single scene is one "
+                    "cube drop with the same total particle count"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (reference kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -201,6 +249,7 @@ def main():
                    "alg_bytes_per_step": b_step},
         "roofline": roofline,
         "kernels": kernels,
+        "batched": None if args.no_batched else batched_leg(args.steps, args.warmup, stream, torch),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(x, v, params),
     }
     print(json.dumps(out), flush=True)

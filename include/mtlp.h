@@ -177,8 +177,43 @@ mtlp_status mtlp_set_profiling(mtlp_engine* e, int enable);
 mtlp_status mtlp_stage_times(mtlp_engine* e, float* ms, const char** names,
                              uint32_t capacity, uint32_t* count);
 /* Device error flags raised by kernels since the last call (bit 0: scan
- * look-back gave up waiting); synchronises and clears them. */
+ * look-back gave up waiting; bit 1: cell offsets inconsistent);
+ * synchronises and clears them. */
 mtlp_status mtlp_error_flags(mtlp_engine* e, uint32_t* flags);
+
+/* ---- Batched ensembles (no reference counterpart: the reference steps one
+ * scene per Engine; these entry points step many independent scenes, each
+ * with its own particle count, bounds/grid, rest density, gravity and other
+ * parameters, with one kernel launch per stage for all of them).  A batch is
+ * an mtlp_engine: mtlp_step, mtlp_upload_state, mtlp_get_buffer and friends
+ * act on all scenes laid end to end (scene s owns particles
+ * [particle_offsets[s], particle_offsets[s+1]) in both original and sorted
+ * order, and cells [cell_offsets[s], cell_offsets[s+1])).  Every scene must
+ * use the same iteration count. */
+mtlp_status mtlp_batch_create(int hip_device, uint32_t n_scenes, const uint32_t* counts,
+                              const mtlp_params* per_scene, mtlp_engine** out);
+/* Seeds scene s with mtlp_generate_scene(scene, counts[s], seed + s,
+ * per_scene[s]); synchronous. */
+mtlp_status mtlp_batch_seed(mtlp_engine* e, uint32_t scene, uint64_t seed);
+/* Scene count and offsets (arrays of n_scenes + 1 entries, may be NULL). */
+mtlp_status mtlp_batch_info(const mtlp_engine* e, uint32_t* n_scenes, uint32_t* particle_offsets,
+                            uint32_t* cell_offsets);
+mtlp_status mtlp_batch_set_params(mtlp_engine* e, uint32_t scene, const mtlp_params* p);
+
+/* n_steps calls of mtlp_step with the same dt. */
+mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n_steps);
+/* With graphs on, mtlp_step captures its launches once per dt into a HIP
+ * graph and replays it (the look-back words of the scan are cleared by a
+ * kernel inside the graph).  Profiling is ignored while graphs are on. */
+mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
+/* Times the stencil kernels at each workgroup size on the current state
+ * (restoring it afterwards) and keeps the fastest; with cache_path, reads a
+ * previous choice for the same batch shape from that text file or appends
+ * the new one.  *block receives the chosen size (may be NULL). */
+mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const char* cache_path,
+                          uint32_t* block);
+/* Workgroup size of the stencil kernels (128, 256 or 512). */
+mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
 
 #ifdef __cplusplus
 }

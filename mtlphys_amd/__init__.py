@@ -7,7 +7,7 @@ the reference's Engine interface.
 """
 from . import _capi as capi
 from ._capi import MtlpError, Params, default_params, grid_dims, scene_params
-from .engine import DeviceBuffer, Engine, generate_scene
+from .engine import Batch, DeviceBuffer, Engine, generate_scene
 
-__all__ = ["capi", "Engine", "DeviceBuffer", "MtlpError", "Params", "default_params",
+__all__ = ["capi", "Engine", "Batch", "DeviceBuffer", "MtlpError", "Params", "default_params",
            "scene_params", "grid_dims", "generate_scene"]

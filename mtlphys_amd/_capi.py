@@ -101,6 +101,15 @@ SIGNATURES = {
     "mtlp_stage_times": (_I, [_P, ctypes.POINTER(_F), ctypes.POINTER(ctypes.c_char_p), _U32,
                               ctypes.POINTER(_U32)]),
     "mtlp_error_flags": (_I, [_P, ctypes.POINTER(_U32)]),
+    "mtlp_batch_create": (_I, [_I, _U32, ctypes.POINTER(_U32), ctypes.POINTER(Params),
+                               ctypes.POINTER(_P)]),
+    "mtlp_batch_seed": (_I, [_P, _U32, ctypes.c_uint64]),
+    "mtlp_batch_info": (_I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "mtlp_batch_set_params": (_I, [_P, _U32, ctypes.POINTER(Params)]),
+    "mtlp_step_n": (_I, [_P, _P, _F, _U32]),
+    "mtlp_set_graph": (_I, [_P, _I]),
+    "mtlp_autotune": (_I, [_P, _P, _F, ctypes.c_char_p, ctypes.POINTER(_U32)]),
+    "mtlp_set_stencil_block": (_I, [_P, _U32]),
 }
 
 _lib = None

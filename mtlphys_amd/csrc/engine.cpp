@@ -4,13 +4,18 @@
 // Restates the orchestration of mtlphys::Engine (engine/src/Engine.cpp):
 // buffer allocation (:381-452), parameter derivation (:551-572), the spatial
 // hash (:601-679), the step (:685-811) and the standalone scan (:843-882).
-// Every stage is enqueued on the caller's stream with its constants passed
-// by value; nothing inside a step waits for the device.
+// Every stage is enqueued on the caller's stream; nothing inside a step
+// waits for the device.  An engine holds one or more independent scenes laid
+// end to end (a batch); every stage is one launch for all of them.
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -35,6 +40,12 @@ mtlp_status fail(mtlp_status s, const std::string& msg) {
             return fail(MTLP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
     } while (0)
 
+#define MTLP_TRY(expr)                          \
+    do {                                        \
+        const mtlp_status st_ = (expr);         \
+        if (st_ != MTLP_OK) return st_;         \
+    } while (0)
+
 struct DeviceGuard {
     int prev = -1;
     bool ok = false;
@@ -51,6 +62,7 @@ constexpr uint32_t kPad = 256;  // particle buffers padded like Engine.cpp:383
 constexpr uint32_t kMaxIterations = 256;
 constexpr uint32_t kMaxScorrN = 64;
 constexpr uint32_t kEpochLimit = (1u << 30) - 1u;
+constexpr uint32_t kMaxScenes = 1u << 20;
 
 uint32_t round_up(uint32_t n, uint32_t m) { return ((n + m - 1) / m) * m; }
 
@@ -95,34 +107,28 @@ GridDev make_grid(const mtlp_params& p, const uint32_t dims[3]) {
 }
 
 // writePBFParams (Engine.cpp:551-572) plus the kernel-local constants of
-// applyDeltaSorted (Constraints.metal:158-160, 190), all in FP32.
+// applyDeltaSorted (Constraints.metal:158-160, 190): FP32 like the
+// reference, and FP64 for the terms the kernels evaluate in FP64.
 SolverDev make_solver(const mtlp_params& p) {
     const float kPi = 3.14159265358979323846f;
-    const float h = p.h, h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
+    const float h = p.h, h2 = h * h, h3 = h2 * h, h6 = h3 * h3;
     SolverDev s{};
     s.h = h;
     s.h2 = h2;
-    s.poly6 = 315.0f / (64.0f * kPi * h9);
     s.spiky = 45.0f / (kPi * h6);
     s.inv_rho0 = 1.0f / p.rest_density;
     s.eps = p.epsilon;
-    s.w0 = s.poly6 * h2 * h2 * h2;
-    const float dqh = p.scorr_dq * h;
-    const float dq2 = dqh * dqh;
-    const float dd = h2 - dq2;
-    s.inv_wq = 1.0f / (s.poly6 * dd * dd * dd);
-    s.scorr_k = p.scorr_k;
     s.scorr_n = p.scorr_n;
-    s.max_step = p.max_step * h;
+    const double pi = 3.14159265358979323846;
     const double hd = h, h2d = hd * hd, h3d = h2d * hd, h6d = h3d * h3d, h9d = h6d * h3d;
     s.h2_d = h2d;
-    s.poly6_d = 315.0 / (64.0 * 3.14159265358979323846 * h9d);
+    s.poly6_d = 315.0 / (64.0 * pi * h9d);
     s.w0_d = s.poly6_d * h2d * h2d * h2d;
     s.inv_rho0_d = 1.0 / (double)p.rest_density;
     s.h_d = hd;
-    s.spiky_d = 45.0 / (3.14159265358979323846 * h6d);
-    const double dqhd = (double)p.scorr_dq * hd, ddd = h2d - dqhd * dqhd;
-    s.inv_wq_d = 1.0 / (s.poly6_d * ddd * ddd * ddd);
+    s.spiky_d = 45.0 / (pi * h6d);
+    const double dqh = (double)p.scorr_dq * hd, dd = h2d - dqh * dqh;
+    s.inv_wq_d = 1.0 / (s.poly6_d * dd * dd * dd);
     s.scorr_k_d = p.scorr_k;
     s.max_step_d = (double)p.max_step * hd;
     s.bminx = p.bounds_min[0];
@@ -134,7 +140,7 @@ SolverDev make_solver(const mtlp_params& p) {
     return s;
 }
 
-FinalDev make_final(const mtlp_params& p, float dt) {
+FinalDev make_final(const mtlp_params& p) {
     FinalDev f{};
     f.bminx = p.bounds_min[0];
     f.bminy = p.bounds_min[1];
@@ -142,23 +148,12 @@ FinalDev make_final(const mtlp_params& p, float dt) {
     f.bmaxx = p.bounds_max[0];
     f.bmaxy = p.bounds_max[1];
     f.bmaxz = p.bounds_max[2];
-    f.inv_dt = 1.0f / dt;  // Engine.cpp:561
     f.damping = p.damping;
     f.weps = p.wall_eps;
     f.floor_f = p.floor_friction;
     f.wall_f = p.wall_friction;
-    f.vmax = p.vmax * p.h * f.inv_dt;  // Constraints.metal:347
+    f.vmax_h = p.vmax * p.h;  // Constraints.metal:347: 1.0f * h * invDt
     return f;
-}
-
-PredictDev make_predict(const mtlp_params& p, float dt, bool enable) {
-    PredictDev d{};
-    d.gdtx = p.gravity[0] * dt;
-    d.gdty = p.gravity[1] * dt;
-    d.gdtz = p.gravity[2] * dt;
-    d.dt = dt;
-    d.enable = enable ? 1u : 0u;
-    return d;
 }
 
 template <class T>
@@ -177,14 +172,23 @@ void dev_free(T*& p) {
 
 struct mtlp_engine {
     int device = 0;
-    mtlp_params params{};
-    uint32_t dims[3] = {0, 0, 0};
+    // scenes (host)
+    std::vector<mtlp_params> scene_params;
+    std::vector<std::array<uint32_t, 3>> scene_dims;
+    std::vector<uint32_t> scene_counts;
+    std::vector<uint32_t> p_off;     // n_scenes + 1
+    std::vector<uint32_t> cell_off;  // n_scenes + 1
+    bool batch = false;
     uint32_t cells = 0;
     uint32_t n = 0;
     uint32_t n_alloc = 0;
     uint32_t scene = MTLP_SCENE_CUBE_DROP;
     bool allocated = false;
+    uint32_t stencil_block = 256;
 
+    // scene table (device)
+    SceneDev* d_scenes = nullptr;
+    uint32_t* d_p_off = nullptr;
     // particle buffers (original order)
     float4* x = nullptr;
     float4* v = nullptr;
@@ -213,6 +217,19 @@ struct mtlp_engine {
     std::vector<const char*> stage_names;
     uint32_t stages_recorded = 0;
 
+    // graph replay
+    bool use_graph = false;
+    bool capturing = false;
+    hipStream_t capture_stream = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    float graph_dt = 0.0f;
+
+    uint32_t iterations() const { return scene_params.empty() ? 0u : scene_params[0].iterations; }
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        graph_exec = nullptr;
+    }
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(tmp_index); dev_free(sorted_index);
@@ -225,9 +242,13 @@ struct mtlp_engine {
     }
     ~mtlp_engine() {
         DeviceGuard g(device);
+        drop_graph();
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
         free_particles();
         free_cells();
         dev_free(misc);
+        dev_free(d_scenes);
+        dev_free(d_p_off);
         for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     }
 
@@ -235,20 +256,19 @@ struct mtlp_engine {
         HashBufs b{};
         b.x = x; b.v = v; b.keys = keys; b.tickets = tickets; b.counts = counts;
         b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
-        b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0;
-        b.scan_state = scan_state; b.scan_ticket = misc + 1; b.err = misc + 2;
+        b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         return b;
     }
 
-    uint32_t next_epoch(hipStream_t s, mtlp_status* st) {
-        *st = MTLP_OK;
-        if (scan_epoch >= kEpochLimit) {
-            const hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(scan_state), 0u,
-                                                 2 * (size_t)scan_tiles(cells + 1), s);
-            if (e != hipSuccess) *st = fail(MTLP_E_HIP, hipGetErrorString(e));
-            scan_epoch = 0;
-        }
-        return ++scan_epoch;
+    StepArgs step_args(float dt, bool predict) const {
+        StepArgs a{};
+        a.scenes = d_scenes;
+        a.p_off = d_p_off;
+        a.n_scenes = (uint32_t)scene_params.size();
+        a.n = n;
+        a.dt = dt;
+        a.predict = predict ? 1u : 0u;
+        return a;
     }
 };
 
@@ -258,16 +278,17 @@ namespace {
 struct StageClock {
     mtlp_engine* e;
     hipStream_t s;
+    bool on;
     uint32_t k = 0;
-    StageClock(mtlp_engine* eng, hipStream_t stream) : e(eng), s(stream) {
-        if (!e->profiling) return;
+    StageClock(mtlp_engine* eng, hipStream_t stream)
+        : e(eng), s(stream), on(eng->profiling && !eng->capturing) {
+        if (!on) return;
         e->stage_names.clear();
         e->stages_recorded = 0;
         (void)hipEventRecord(e->events[0], s);
     }
     void mark(const char* name) {
-        if (!e->profiling) return;
-        if (k + 1 >= e->events.size()) return;
+        if (!on || k + 1 >= e->events.size()) return;
         ++k;
         (void)hipEventRecord(e->events[k], s);
         e->stage_names.push_back(name);
@@ -275,9 +296,60 @@ struct StageClock {
     }
 };
 
-mtlp_status alloc_cells(mtlp_engine* e, const uint32_t dims[3]) {
+// Scene layout from per-scene params and counts: grids, cell and particle
+// offsets.  Validates that every scene shares the iteration count.
+mtlp_status layout_scenes(mtlp_engine* e) {
+    const size_t ns = e->scene_params.size();
+    e->scene_dims.assign(ns, {0, 0, 0});
+    e->p_off.assign(ns + 1, 0);
+    e->cell_off.assign(ns + 1, 0);
+    uint64_t cells = 0, parts = 0;
+    for (size_t s = 0; s < ns; ++s) {
+        uint32_t d[3];
+        MTLP_TRY(validate_params(&e->scene_params[s], d));
+        if (e->scene_params[s].iterations != e->scene_params[0].iterations)
+            return fail(MTLP_E_INVALID, "every scene of a batch must use the same iteration count");
+        e->scene_dims[s] = {d[0], d[1], d[2]};
+        e->cell_off[s] = (uint32_t)cells;
+        e->p_off[s] = (uint32_t)parts;
+        cells += (uint64_t)d[0] * d[1] * d[2];
+        parts += e->scene_counts[s];
+        if (cells + 1 > (uint64_t)MTLP_MAX_SCAN_LENGTH)
+            return fail(MTLP_E_RANGE, "total cell count above what the scan supports");
+        if (parts > (uint64_t)MTLP_MAX_PARTICLES)
+            return fail(MTLP_E_RANGE, "particle count above MTLP_MAX_PARTICLES");
+    }
+    e->cell_off[ns] = (uint32_t)cells;
+    e->p_off[ns] = (uint32_t)parts;
+    return MTLP_OK;
+}
+
+mtlp_status upload_scene_table(mtlp_engine* e) {
+    const size_t ns = e->scene_params.size();
+    std::vector<SceneDev> table(ns);
+    for (size_t s = 0; s < ns; ++s) {
+        const mtlp_params& p = e->scene_params[s];
+        SceneDev& d = table[s];
+        d.sd = make_solver(p);
+        d.g = make_grid(p, e->scene_dims[s].data());
+        d.fd = make_final(p);
+        d.gx = p.gravity[0];
+        d.gy = p.gravity[1];
+        d.gz = p.gravity[2];
+        d.cell_off = e->cell_off[s];
+    }
+    dev_free(e->d_scenes);
+    dev_free(e->d_p_off);
+    HIP_TRY(dev_alloc(&e->d_scenes, ns));
+    HIP_TRY(dev_alloc(&e->d_p_off, ns + 1));
+    HIP_TRY(hipMemcpy(e->d_scenes, table.data(), sizeof(SceneDev) * ns, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(e->d_p_off, e->p_off.data(), sizeof(uint32_t) * (ns + 1), hipMemcpyHostToDevice));
+    e->drop_graph();
+    return MTLP_OK;
+}
+
+mtlp_status alloc_cells(mtlp_engine* e, uint32_t C) {
     e->free_cells();
-    const uint32_t C = dims[0] * dims[1] * dims[2];
     const size_t tiles = scan_tiles(C + 1);
     HIP_TRY(dev_alloc(&e->counts, (size_t)C + 1));
     HIP_TRY(dev_alloc(&e->cell_start, (size_t)C + 1));
@@ -285,8 +357,7 @@ mtlp_status alloc_cells(mtlp_engine* e, const uint32_t dims[3]) {
     HIP_TRY(hipMemset(e->counts, 0, sizeof(uint32_t) * ((size_t)C + 1)));
     HIP_TRY(hipMemset(e->cell_start, 0, sizeof(uint32_t) * ((size_t)C + 1)));
     HIP_TRY(hipMemset(e->scan_state, 0, sizeof(uint64_t) * tiles));
-    e->scan_epoch = 0;
-    std::memcpy(e->dims, dims, sizeof(e->dims));
+    e->scan_epoch = 1;  // eager launches use epochs >= 2; graph replays use 1
     e->cells = C;
     return MTLP_OK;
 }
@@ -320,27 +391,136 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     return MTLP_OK;
 }
 
+// (Re)allocates every buffer for the current scene list.
+mtlp_status allocate_all(mtlp_engine* e) {
+    DeviceGuard g(e->device);
+    HIP_TRY(hipDeviceSynchronize());
+    e->allocated = false;
+    MTLP_TRY(layout_scenes(e));
+    MTLP_TRY(alloc_particles(e, e->p_off.back()));
+    MTLP_TRY(alloc_cells(e, e->cell_off.back()));
+    MTLP_TRY(upload_scene_table(e));
+    e->final_buf = 0;
+    e->allocated = true;
+    return MTLP_OK;
+}
+
+// Seeds every scene with the generator of `scene`; synchronous.
+mtlp_status seed_scenes(mtlp_engine* e, uint32_t scene, uint64_t seed) {
+    const size_t ns = e->scene_params.size();
+    std::vector<float> hx(4 * (size_t)e->n), hv(4 * (size_t)e->n);
+    for (size_t s = 0; s < ns; ++s) {
+        const size_t off = 4 * (size_t)e->p_off[s];
+        const mtlp_status st = generate_scene(scene, e->scene_counts[s], seed + (ns > 1 ? s : 0),
+                                              e->scene_params[s], hx.data() + off, hv.data() + off);
+        if (st != MTLP_OK) return fail(st, "unknown scene id");
+    }
+    DeviceGuard g(e->device);
+    if (e->n) {
+        HIP_TRY(hipMemcpy(e->x, hx.data(), sizeof(float) * 4 * (size_t)e->n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->v, hv.data(), sizeof(float) * 4 * (size_t)e->n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->prev, hx.data(), sizeof(float) * 4 * (size_t)e->n, hipMemcpyHostToDevice));
+    }
+    e->scene = scene;
+    return MTLP_OK;
+}
+
 // Spatial hash into sorted[0]: predict (optional) + hash + count, scan,
 // stable scatter + gather.  Engine.cpp:601-667 and :715-726.
-mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const PredictDev& pd, StageClock& clk) {
-    const GridDev g = make_grid(e->params, e->dims);
+mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, StageClock& clk) {
     HashBufs b = e->hash_bufs();
-    HIP_TRY(launch_predict_hash_count(b, e->n, pd, g, s));
+    HIP_TRY(launch_predict_hash_count(b, a, s));
     clk.mark("predict_hash_count");
-    mtlp_status st;
-    const uint32_t epoch = e->next_epoch(s, &st);
-    if (st != MTLP_OK) return st;
+    uint32_t epoch;
+    const uint32_t tiles = scan_tiles(e->cells + 1);
+    if (e->capturing) {
+        // a replayed graph reuses its epoch: clear the look-back words first
+        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(e->scan_state), 0u, 2 * (size_t)tiles, s));
+        epoch = 1;
+    } else {
+        if (e->scan_epoch >= kEpochLimit) {
+            HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(e->scan_state), 0u, 2 * (size_t)tiles, s));
+            e->scan_epoch = 1;  // epoch 1 is what graph replays use
+        }
+        epoch = ++e->scan_epoch;
+    }
     HIP_TRY(launch_exclusive_scan(e->counts, e->cell_start, e->cells + 1, true, e->scan_state,
                                   e->misc + 1, epoch, e->misc + 2, s));
     clk.mark("exclusive_scan");
-    HIP_TRY(launch_scatter_gather(b, e->n, pd, s));
+    HIP_TRY(launch_scatter_gather(b, a, s));
     clk.mark("stable_scatter_gather");
+    return MTLP_OK;
+}
+
+// Engine::step (Engine.cpp:685-811) for every scene.
+mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
+    StageClock clk(e, s);
+    const StepArgs a = e->step_args(dt, true);
+    MTLP_TRY(enqueue_hash(e, s, a, clk));
+    const uint32_t iters = e->iterations();
+    int cur = 0;
+    for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
+        const bool last = it + 1 == iters;
+        HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a,
+                                      last ? e->lambdas : nullptr, it == 0 ? e->nbr : nullptr,
+                                      e->stencil_block, s));
+        clk.mark("density_lambda");
+        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1],
+                                   e->stencil_block, s));
+        clk.mark("apply_delta");
+        cur ^= 1;
+    }
+    if (iters == 0) {
+        HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
+        clk.mark("count_neighbors");
+    }
+    HIP_TRY(launch_finalize(e->sorted[cur], e->sorted_index, a, e->x, e->v, e->prev, s));
+    clk.mark("scatter_finalize");
+    e->final_buf = cur;
+    return MTLP_OK;
+}
+
+mtlp_status capture_graph(mtlp_engine* e, float dt) {
+    e->drop_graph();
+    if (!e->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
+    e->capturing = true;
+    const mtlp_status st = enqueue_step(e, e->capture_stream, dt);
+    e->capturing = false;
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(e->capture_stream, &graph);
+    if (st != MTLP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    HIP_TRY(end);
+    const hipError_t inst = hipGraphInstantiate(&e->graph_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_TRY(inst);
+    e->graph_dt = dt;
     return MTLP_OK;
 }
 
 mtlp_status check_engine(const mtlp_engine* e) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
     if (!e->allocated) return fail(MTLP_E_INVALID, "engine has no buffers: call mtlp_reset first");
+    return MTLP_OK;
+}
+
+mtlp_status new_engine(int hip_device, mtlp_engine** out) {
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (hip_device < 0 || hip_device >= count) return fail(MTLP_E_INVALID, "no such HIP device");
+    mtlp_engine* e = new (std::nothrow) mtlp_engine();
+    if (!e) return fail(MTLP_E_NOMEM, "host allocation failed");
+    e->device = hip_device;
+    DeviceGuard g(hip_device);
+    if (!g.ok || hipMalloc(reinterpret_cast<void**>(&e->misc), 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(e->misc, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
+        delete e;
+        return fail(MTLP_E_NOMEM, "device allocation failed");
+    }
+    *out = e;
     return MTLP_OK;
 }
 
@@ -394,23 +574,33 @@ mtlp_status mtlp_create(int hip_device, const mtlp_params* p, mtlp_engine** out)
     mtlp_params params;
     if (p) params = *p; else default_params(&params);
     uint32_t dims[3];
-    mtlp_status st = validate_params(&params, dims);
-    if (st != MTLP_OK) return st;
-    int count = 0;
-    HIP_TRY(hipGetDeviceCount(&count));
-    if (hip_device < 0 || hip_device >= count) return fail(MTLP_E_INVALID, "no such HIP device");
-    mtlp_engine* e = new (std::nothrow) mtlp_engine();
-    if (!e) return fail(MTLP_E_NOMEM, "host allocation failed");
-    e->device = hip_device;
-    e->params = params;
-    std::memcpy(e->dims, dims, sizeof(dims));
-    DeviceGuard g(hip_device);
-    if (!g.ok) { delete e; return fail(MTLP_E_HIP, "hipSetDevice failed"); }
-    if (hipMalloc(reinterpret_cast<void**>(&e->misc), 4 * sizeof(uint32_t)) != hipSuccess ||
-        hipMemset(e->misc, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
+    MTLP_TRY(validate_params(&params, dims));
+    mtlp_engine* e = nullptr;
+    MTLP_TRY(new_engine(hip_device, &e));
+    e->scene_params = {params};
+    e->scene_counts = {0};
+    *out = e;
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_batch_create(int hip_device, uint32_t n_scenes, const uint32_t* counts,
+                              const mtlp_params* per_scene, mtlp_engine** out) {
+    if (!out) return fail(MTLP_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (n_scenes == 0 || !counts || !per_scene) return fail(MTLP_E_INVALID, "empty batch");
+    if (n_scenes > kMaxScenes) return fail(MTLP_E_RANGE, "too many scenes");
+    mtlp_engine* e = nullptr;
+    MTLP_TRY(new_engine(hip_device, &e));
+    e->batch = true;
+    e->scene_params.assign(per_scene, per_scene + n_scenes);
+    e->scene_counts.assign(counts, counts + n_scenes);
+    const mtlp_status st = allocate_all(e);
+    if (st != MTLP_OK) {
+        const std::string msg = g_last_error;
         delete e;
-        return fail(MTLP_E_NOMEM, "device allocation failed");
+        return fail(st, msg);
     }
+    e->scene = MTLP_SCENE_EMPTY;
     *out = e;
     return MTLP_OK;
 }
@@ -428,56 +618,80 @@ mtlp_status mtlp_destroy(mtlp_engine* e) {
 mtlp_status mtlp_reset(mtlp_engine* e, uint32_t n, uint32_t scene, uint64_t seed) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
     if (n > MTLP_MAX_PARTICLES) return fail(MTLP_E_RANGE, "particle count above MTLP_MAX_PARTICLES");
-    std::vector<float> hx(4 * (size_t)n), hv(4 * (size_t)n);
-    mtlp_status st = generate_scene(scene, n, seed, e->params, hx.data(), hv.data());
-    if (st != MTLP_OK) return fail(st, "unknown scene id");
+    mtlp_params probe;
+    if (scene_params(scene, 0, &probe) != MTLP_OK) return fail(MTLP_E_INVALID, "unknown scene id");
+    if (e->batch) {
+        if (n != e->n) return fail(MTLP_E_RANGE, "a batch keeps its particle count; use mtlp_batch_seed");
+    } else {
+        e->scene_counts = {n};
+        MTLP_TRY(allocate_all(e));
+    }
+    return seed_scenes(e, scene, seed);
+}
+
+mtlp_status mtlp_batch_seed(mtlp_engine* e, uint32_t scene, uint64_t seed) {
+    MTLP_TRY(check_engine(e));
+    return seed_scenes(e, scene, seed);
+}
+
+mtlp_status mtlp_batch_info(const mtlp_engine* e, uint32_t* n_scenes, uint32_t* particle_offsets,
+                            uint32_t* cell_offsets) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (n_scenes) *n_scenes = (uint32_t)e->scene_params.size();
+    if (particle_offsets && !e->p_off.empty())
+        std::memcpy(particle_offsets, e->p_off.data(), sizeof(uint32_t) * e->p_off.size());
+    if (cell_offsets && !e->cell_off.empty())
+        std::memcpy(cell_offsets, e->cell_off.data(), sizeof(uint32_t) * e->cell_off.size());
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_batch_set_params(mtlp_engine* e, uint32_t scene, const mtlp_params* p) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (scene >= e->scene_params.size()) return fail(MTLP_E_RANGE, "no such scene");
+    uint32_t dims[3];
+    MTLP_TRY(validate_params(p, dims));
+    const mtlp_params old = e->scene_params[scene];
+    e->scene_params[scene] = *p;
+    if (!e->allocated) return MTLP_OK;
+    const auto& d = e->scene_dims[scene];
+    if (dims[0] != d[0] || dims[1] != d[1] || dims[2] != d[2]) {
+        // new grid: re-lay the cells (particle buffers keep their contents)
+        DeviceGuard g(e->device);
+        HIP_TRY(hipDeviceSynchronize());
+        mtlp_status st = layout_scenes(e);
+        if (st == MTLP_OK) st = alloc_cells(e, e->cell_off.back());
+        if (st != MTLP_OK) {
+            e->scene_params[scene] = old;
+            e->allocated = false;
+            return st;
+        }
+    } else {
+        mtlp_status st = layout_scenes(e);  // checks the shared iteration count
+        if (st != MTLP_OK) {
+            e->scene_params[scene] = old;
+            (void)layout_scenes(e);
+            return st;
+        }
+    }
     DeviceGuard g(e->device);
     HIP_TRY(hipDeviceSynchronize());
-    e->allocated = false;
-    st = alloc_particles(e, n);
-    if (st != MTLP_OK) return st;
-    const uint32_t dims[3] = {e->dims[0], e->dims[1], e->dims[2]};
-    st = alloc_cells(e, dims);
-    if (st != MTLP_OK) return st;
-    if (n) {
-        HIP_TRY(hipMemcpy(e->x, hx.data(), sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(e->v, hv.data(), sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(e->prev, hx.data(), sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
-    }
-    e->scene = scene;
-    e->final_buf = 0;
-    e->allocated = true;
-    return MTLP_OK;
+    return upload_scene_table(e);
 }
 
 mtlp_status mtlp_set_params(mtlp_engine* e, const mtlp_params* p) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
-    uint32_t dims[3];
-    mtlp_status st = validate_params(p, dims);
-    if (st != MTLP_OK) return st;
-    const bool regrid = dims[0] != e->dims[0] || dims[1] != e->dims[1] || dims[2] != e->dims[2];
-    e->params = *p;
-    if (regrid) {
-        std::memcpy(e->dims, dims, sizeof(dims));
-        if (e->allocated) {
-            DeviceGuard g(e->device);
-            HIP_TRY(hipDeviceSynchronize());
-            st = alloc_cells(e, dims);
-            if (st != MTLP_OK) { e->allocated = false; return st; }
-        }
-    }
-    return MTLP_OK;
+    if (e->batch) return fail(MTLP_E_INVALID, "use mtlp_batch_set_params on a batch");
+    return mtlp_batch_set_params(e, 0, p);
 }
 
 mtlp_status mtlp_get_params(const mtlp_engine* e, mtlp_params* out) {
     if (!e || !out) return fail(MTLP_E_INVALID, "NULL argument");
-    *out = e->params;
+    *out = e->scene_params[0];
     return MTLP_OK;
 }
 
 mtlp_status mtlp_upload_state(mtlp_engine* e, const float* x4, const float* v4, uint32_t n) {
-    mtlp_status st = check_engine(e);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(check_engine(e));
     if (n != e->n) return fail(MTLP_E_RANGE, "n differs from the engine's particle count");
     if (n == 0) return MTLP_OK;
     DeviceGuard g(e->device);
@@ -488,55 +702,139 @@ mtlp_status mtlp_upload_state(mtlp_engine* e, const float* x4, const float* v4, 
 }
 
 mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt) {
-    mtlp_status st = check_engine(e);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(check_engine(e));
     if (std::isnan(dt) || std::isinf(dt)) return fail(MTLP_E_INVALID, "dt is not finite");
     if (dt <= 0.0f) return MTLP_OK;  // Engine.cpp:686
     if (e->n == 0) return MTLP_OK;
     DeviceGuard g(e->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    StageClock clk(e, s);
-    const mtlp_params& p = e->params;
-    const GridDev grid = make_grid(p, e->dims);
-    const SolverDev sd = make_solver(p);
+    if (e->use_graph) {
+        if (!e->graph_exec || e->graph_dt != dt) MTLP_TRY(capture_graph(e, dt));
+        HIP_TRY(hipGraphLaunch(e->graph_exec, s));
+        e->final_buf = (int)(e->iterations() & 1u);
+        return MTLP_OK;
+    }
+    return enqueue_step(e, s, dt);
+}
 
-    st = enqueue_hash(e, s, make_predict(p, dt, true), clk);
+mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n_steps) {
+    for (uint32_t k = 0; k < n_steps; ++k) MTLP_TRY(mtlp_step(e, stream, dt));
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_set_graph(mtlp_engine* e, int enable) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    DeviceGuard g(e->device);
+    e->use_graph = enable != 0;
+    if (!e->use_graph) e->drop_graph();
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    bool ok = false;
+    for (uint32_t b : kStencilBlocks) ok |= b == block;
+    if (!ok) return fail(MTLP_E_INVALID, "stencil block must be 128, 256 or 512");
+    if (e->stencil_block != block) e->drop_graph();
+    e->stencil_block = block;
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const char* cache_path,
+                          uint32_t* block) {
+    MTLP_TRY(check_engine(e));
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MTLP_E_INVALID, "dt must be > 0");
+    std::ostringstream key;
+    key << "n=" << e->n << ",scenes=" << e->scene_params.size() << ",cells=" << e->cells
+        << ",iters=" << e->iterations();
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string k;
+        uint32_t b;
+        while (in >> k >> b) {
+            if (k == key.str() && mtlp_set_stencil_block(e, b) == MTLP_OK) {
+                if (block) *block = b;
+                return MTLP_OK;
+            }
+        }
+    }
+    if (e->n == 0) {
+        if (block) *block = e->stencil_block;
+        return MTLP_OK;
+    }
+    DeviceGuard g(e->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // time on a copy of the state, then put it back
+    float4 *sx = nullptr, *sv = nullptr, *sp = nullptr;
+    HIP_TRY(dev_alloc(&sx, e->n));
+    HIP_TRY(dev_alloc(&sv, e->n));
+    HIP_TRY(dev_alloc(&sp, e->n));
+    HIP_TRY(hipMemcpyAsync(sx, e->x, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sv, e->v, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sp, e->prev, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
+    const bool graph = e->use_graph, prof = e->profiling;
+    e->use_graph = false;
+    e->profiling = false;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    uint32_t best = e->stencil_block;
+    float best_ms = 1e30f;
+    mtlp_status st = MTLP_OK;
+    for (uint32_t b : kStencilBlocks) {
+        e->stencil_block = b;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && st == MTLP_OK; ++rep) {  // rep 0 warms up
+            if (hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipEventRecord(t0, s) != hipSuccess) {
+                st = fail(MTLP_E_HIP, "autotune copy/event failed");
+                break;
+            }
+            st = enqueue_step(e, s, dt);
+            if (st != MTLP_OK || hipEventRecord(t1, s) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+                if (st == MTLP_OK) st = fail(MTLP_E_HIP, "autotune timing failed");
+                break;
+            }
+        }
+        if (st != MTLP_OK) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = b;
+        }
+    }
+    (void)hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(e->prev, sp, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    dev_free(sx);
+    dev_free(sv);
+    dev_free(sp);
+    e->use_graph = graph;
+    e->profiling = prof;
+    e->stencil_block = best;
+    e->drop_graph();
     if (st != MTLP_OK) return st;
-
-    int cur = 0;
-    for (uint32_t it = 0; it < p.iterations; ++it) {  // Engine.cpp:732-774
-        const bool last = it + 1 == p.iterations;
-        HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, e->n, grid, sd,
-                                      last ? e->lambdas : nullptr, it == 0 ? e->nbr : nullptr, s));
-        clk.mark("density_lambda");
-        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, e->n, grid, sd, e->sorted[cur ^ 1], s));
-        clk.mark("apply_delta");
-        cur ^= 1;
+    if (cache_path) {
+        std::ofstream out(cache_path, std::ios::app);
+        out << key.str() << " " << best << "\n";
     }
-    if (p.iterations == 0) {
-        HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, e->n, grid,
-                                       p.h * p.h, e->nbr, s));
-        clk.mark("count_neighbors");
-    }
-    HIP_TRY(launch_finalize(e->sorted[cur], e->sorted_index, e->n, make_final(p, dt), e->x, e->v,
-                            e->prev, s));
-    clk.mark("scatter_finalize");
-    e->final_buf = cur;
+    if (block) *block = best;
     return MTLP_OK;
 }
 
 mtlp_status mtlp_build_spatial_hash(mtlp_engine* e, mtlp_stream stream) {
-    mtlp_status st = check_engine(e);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(check_engine(e));
     if (e->n == 0) return MTLP_OK;
     DeviceGuard g(e->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     StageClock clk(e, s);
-    st = enqueue_hash(e, s, make_predict(e->params, 0.0f, false), clk);
-    if (st != MTLP_OK) return st;
-    // neighborRadiusSq = kSmoothingH^2 (Engine.cpp:447)
-    HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, e->n,
-                                   make_grid(e->params, e->dims), e->params.h * e->params.h, e->nbr, s));
+    const StepArgs a = e->step_args(0.0f, false);
+    MTLP_TRY(enqueue_hash(e, s, a, clk));
+    HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
     clk.mark("count_neighbors");
     e->final_buf = 0;
     return MTLP_OK;
@@ -570,8 +868,7 @@ mtlp_status mtlp_exclusive_scan(mtlp_engine* e, mtlp_stream stream, const uint32
 }
 
 mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, size_t* bytes) {
-    mtlp_status st = check_engine(e);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(check_engine(e));
     if (!d_ptr) return fail(MTLP_E_INVALID, "d_ptr is NULL");
     void* p = nullptr;
     size_t b = 0;
@@ -596,8 +893,7 @@ mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, siz
 mtlp_status mtlp_read_buffer(mtlp_engine* e, mtlp_buffer which, void* host_dst, size_t bytes) {
     void* p = nullptr;
     size_t b = 0;
-    mtlp_status st = mtlp_get_buffer(e, which, &p, &b);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(mtlp_get_buffer(e, which, &p, &b));
     if (bytes > b) return fail(MTLP_E_RANGE, "read larger than the buffer");
     if (bytes == 0) return MTLP_OK;
     if (!host_dst) return fail(MTLP_E_INVALID, "host_dst is NULL");
@@ -610,8 +906,7 @@ mtlp_status mtlp_read_buffer(mtlp_engine* e, mtlp_buffer which, void* host_dst, 
 mtlp_status mtlp_write_buffer(mtlp_engine* e, mtlp_buffer which, const void* host_src, size_t bytes) {
     void* p = nullptr;
     size_t b = 0;
-    mtlp_status st = mtlp_get_buffer(e, which, &p, &b);
-    if (st != MTLP_OK) return st;
+    MTLP_TRY(mtlp_get_buffer(e, which, &p, &b));
     if (bytes > b) return fail(MTLP_E_RANGE, "write larger than the buffer");
     if (bytes == 0) return MTLP_OK;
     if (!host_src) return fail(MTLP_E_INVALID, "host_src is NULL");
@@ -622,8 +917,15 @@ mtlp_status mtlp_write_buffer(mtlp_engine* e, mtlp_buffer which, const void* hos
 }
 
 uint32_t mtlp_particle_count(const mtlp_engine* e) { return e ? e->n : 0u; }
-uint32_t mtlp_total_cells(const mtlp_engine* e) { return e ? e->dims[0] * e->dims[1] * e->dims[2] : 0u; }
-float mtlp_cell_size(const mtlp_engine* e) { return e ? e->params.cell_size : 0.0f; }
+
+uint32_t mtlp_total_cells(const mtlp_engine* e) {
+    if (!e) return 0u;
+    if (e->allocated) return e->cells;
+    uint32_t d[3];
+    return validate_params(&e->scene_params[0], d) == MTLP_OK ? d[0] * d[1] * d[2] : 0u;
+}
+
+float mtlp_cell_size(const mtlp_engine* e) { return e ? e->scene_params[0].cell_size : 0.0f; }
 
 mtlp_status mtlp_set_profiling(mtlp_engine* e, int enable) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");

@@ -1,6 +1,7 @@
 // hash.hip -- spatial hash for gfx950: fused predict + cell hash + atomic
 // count, stable scatter into cell order, gather of sorted positions, and
-// the neighbour count of buildSpatialHash.
+// the neighbour count of buildSpatialHash.  One launch per stage covers every
+// scene of a batch.
 //
 // Reference stages (Engine.cpp:601-679, 715-723): predictPositions,
 // hashCells, countCells, [scan], scatterParticles, gatherPositions,
@@ -21,15 +22,19 @@ namespace mtlp {
 namespace {
 
 __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
-    const float4* __restrict__ x, const float4* __restrict__ v, uint32_t n, PredictDev pd,
-    GridDev g, uint32_t* __restrict__ keys, uint32_t* __restrict__ tickets,
-    uint32_t* __restrict__ counts, uint32_t* __restrict__ big_count) {
+    const float4* __restrict__ x, const float4* __restrict__ v, StepArgs a,
+    uint32_t* __restrict__ keys, uint32_t* __restrict__ tickets, uint32_t* __restrict__ counts,
+    uint32_t* __restrict__ big_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) *big_count = 0u;  // the big-cell list of this build starts empty
-    if (i >= n) return;
-    const float4 vi = pd.enable ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 p = predict_pos(x[i], vi, pd);
-    const uint32_t key = key_of(cell_of(p.x, p.y, p.z, g), g);
+    if (i >= a.n) return;
+    const float4 xi = x[i];
+    const float4 vi = a.predict ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t key;
+    with_scene(i, a, [&](const SceneDev& sc) {
+        const float4 p = a.predict ? predict_pos(xi, vi, sc.gx, sc.gy, sc.gz, a.dt) : xi;
+        key = sc.cell_off + key_of(cell_of(p.x, p.y, p.z, sc.g), sc.g);
+    });
     keys[i] = key;
     tickets[i] = __hip_atomic_fetch_add(&counts[key], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -51,10 +56,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter_ticket(
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
-    const float4* __restrict__ v, uint32_t n, PredictDev pd, uint32_t* __restrict__ sorted_index,
+    const float4* __restrict__ v, StepArgs a, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted, uint32_t* __restrict__ big_list, uint32_t* __restrict__ big_count,
     uint32_t* __restrict__ err) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = a.n;
     if (s >= n) return;
     const uint32_t i = tmp[s];
     if (i >= n) {
@@ -62,54 +68,45 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
         return;
     }
     const uint32_t key = keys[i];
-    const uint32_t a = cell_start[key], b = cell_start[key + 1];
-    if (b > n || a > b) {
+    const uint32_t lo = cell_start[key], hi = cell_start[key + 1];
+    if (hi > n || lo > hi) {
         atomicOr(err, kErrHashRange);
         return;
     }
-    if (b - a > kSmallCell) {
-        if (s == a) big_list[atomicAdd(big_count, 1u)] = key;
+    if (hi - lo > kSmallCell) {
+        if (s == lo) big_list[atomicAdd(big_count, 1u)] = key;
         return;
     }
     uint32_t rank = 0;
-    for (uint32_t m = a; m < b; ++m) rank += (tmp[m] < i) ? 1u : 0u;
-    const uint32_t dst = a + rank;
+    for (uint32_t m = lo; m < hi; ++m) rank += (tmp[m] < i) ? 1u : 0u;
+    const uint32_t dst = lo + rank;
     sorted_index[dst] = i;
-    const float4 vi = pd.enable ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    sorted[dst] = predict_pos(x[i], vi, pd);
-}
-
-__device__ __forceinline__ void emit_slot(uint32_t dst, uint32_t i, const float4* __restrict__ x,
-                                          const float4* __restrict__ v, const PredictDev& pd,
-                                          uint32_t* __restrict__ sorted_index,
-                                          float4* __restrict__ sorted) {
-    sorted_index[dst] = i;
-    const float4 vi = pd.enable ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    sorted[dst] = predict_pos(x[i], vi, pd);
+    sorted[dst] = binned_pos(x, v, i, a);
 }
 
 // One workgroup per listed cell (grid-stride over the list).
 __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
-    const float4* __restrict__ v, uint32_t n, PredictDev pd, const uint32_t* __restrict__ big_list,
+    const float4* __restrict__ v, StepArgs a, const uint32_t* __restrict__ big_list,
     const uint32_t* __restrict__ big_count, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted) {
     __shared__ uint32_t lds[kLdsSortMax];
     __shared__ uint32_t s_wave[kBigBlock / 64];
     const uint32_t t = threadIdx.x;
     const uint32_t lane = t & 63u, wave = t >> 6;
+    const uint32_t n = a.n;
     const uint32_t count = *big_count;
     for (uint32_t bi = blockIdx.x; bi < count; bi += gridDim.x) {
         const uint32_t key = big_list[bi];
-        const uint32_t a = cell_start[key], b = cell_start[key + 1];
-        if (b > n || a > b) continue;  // flagged by k_rank_gather
-        const uint32_t len = b - a;
+        const uint32_t lo = cell_start[key], hi = cell_start[key + 1];
+        if (hi > n || lo > hi) continue;  // flagged by k_rank_gather
+        const uint32_t len = hi - lo;
         if (len <= kLdsSortMax) {
             uint32_t p2 = 1;
             while (p2 < len) p2 <<= 1;
             for (uint32_t k = t; k < p2; k += kBigBlock) {
-                const uint32_t idx = (k < len) ? tmp[a + k] : 0xffffffffu;
+                const uint32_t idx = (k < len) ? tmp[lo + k] : 0xffffffffu;
                 lds[k] = (k < len && idx >= n) ? 0u : idx;  // out-of-range ids were flagged
             }
             __syncthreads();
@@ -117,23 +114,27 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
             for (uint32_t kk = 2; kk <= p2; kk <<= 1) {
                 for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
                     for (uint32_t q = t; q < (p2 >> 1); q += kBigBlock) {
-                        const uint32_t lo = 2 * j * (q / j) + (q % j);
-                        const uint32_t hi = lo + j;
-                        const bool up = (lo & kk) == 0;
-                        const uint32_t va = lds[lo], vb = lds[hi];
+                        const uint32_t e0 = 2 * j * (q / j) + (q % j);
+                        const uint32_t e1 = e0 + j;
+                        const bool up = (e0 & kk) == 0;
+                        const uint32_t va = lds[e0], vb = lds[e1];
                         if ((va > vb) == up) {
-                            lds[lo] = vb;
-                            lds[hi] = va;
+                            lds[e0] = vb;
+                            lds[e1] = va;
                         }
                     }
                     __syncthreads();
                 }
             }
-            for (uint32_t k = t; k < len; k += kBigBlock) emit_slot(a + k, lds[k], x, v, pd, sorted_index, sorted);
+            for (uint32_t k = t; k < len; k += kBigBlock) {
+                const uint32_t i = lds[k];
+                sorted_index[lo + k] = i;
+                sorted[lo + k] = binned_pos(x, v, i, a);
+            }
             __syncthreads();
         } else {
             // Rank by an ordered pass over all keys: the r-th particle (by
-            // index) whose key matches goes to slot a + r.
+            // index) whose key matches goes to slot lo + r.
             uint32_t run = 0;
             for (uint32_t base = 0; base < n; base += kBigBlock) {
                 const uint32_t j = base + t;
@@ -148,8 +149,11 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
                 }
                 if (hit) {
                     const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    const uint32_t dst = a + run + pre + below;
-                    if (dst < b) emit_slot(dst, j, x, v, pd, sorted_index, sorted);
+                    const uint32_t dst = lo + run + pre + below;
+                    if (dst < hi) {
+                        sorted_index[dst] = j;
+                        sorted[dst] = binned_pos(x, v, j, a);
+                    }
                 }
                 run += tot;
                 __syncthreads();
@@ -158,15 +162,13 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     }
 }
 
-// countNeighbors (Spatial.metal:207-243), per sorted slot: neighbours j != s
-// in the 27-cell stencil of cell(p_s) with |p_j - p_s|^2 < r2max.  Same
-// arithmetic (no FMA) as the reference's dot(d, d).
-__global__ __launch_bounds__(kBlock) void k_count_neighbors(
-    const float4* __restrict__ sorted, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ sorted_index, uint32_t n, GridDev g, float r2max,
-    uint32_t* __restrict__ nbr) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+__device__ __forceinline__ void count_neighbors_slot(uint32_t s, const SceneDev& sc,
+                                                     const float4* __restrict__ sorted,
+                                                     const uint32_t* __restrict__ cell_start,
+                                                     const uint32_t* __restrict__ sorted_index,
+                                                     uint32_t n, uint32_t* __restrict__ nbr) {
+    const GridDev g = sc.g;
+    const float r2max = sc.sd.h2;  // neighborRadiusSq = kSmoothingH^2, Engine.cpp:447
     const float4 p = sorted[s];
     const Cell3 c = cell_of(p.x, p.y, p.z, g);
     const int x0 = max(c.x - 1, 0), x1 = min(c.x + 1, (int)g.dx - 1);
@@ -175,10 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_count_neighbors(
     uint32_t cnt = 0;
     for (int z = z0; z <= z1; ++z)
         for (int y = y0; y <= y1; ++y) {
-            const uint32_t row = (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
-            const uint32_t a = cell_start[row + (uint32_t)x0];
-            const uint32_t b = min(cell_start[row + (uint32_t)x1 + 1], n);
-            for (uint32_t j = a; j < b; ++j) {
+            const uint32_t row = sc.cell_off + (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
+            const uint32_t lo = cell_start[row + (uint32_t)x0];
+            const uint32_t hi = min(cell_start[row + (uint32_t)x1 + 1], n);
+            for (uint32_t j = lo; j < hi; ++j) {
                 const float4 q = sorted[j];
                 float d2;
                 {
@@ -192,42 +194,52 @@ __global__ __launch_bounds__(kBlock) void k_count_neighbors(
     nbr[sorted_index[s]] = cnt;
 }
 
+// countNeighbors (Spatial.metal:207-243), per sorted slot: neighbours j != s
+// in the 27-cell stencil of cell(p_s) with |p_j - p_s|^2 < h^2.  Same
+// arithmetic (no FMA) as the reference's dot(d, d).
+__global__ __launch_bounds__(kBlock) void k_count_neighbors(
+    const float4* __restrict__ sorted, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ sorted_index, StepArgs a, uint32_t* __restrict__ nbr) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = a.n;
+    if (s >= n) return;
+    with_scene(s, a, [&](const SceneDev& sc) { count_neighbors_slot(s, sc, sorted, cell_start, sorted_index, n, nbr); });
+}
+
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 }  // namespace
 
-hipError_t launch_predict_hash_count(const HashBufs& b, uint32_t n, const PredictDev& pd,
-                                     const GridDev& g, hipStream_t s) {
-    const uint32_t grid = blocks_for(n > 0 ? n : 1, kBlock);
-    hipLaunchKernelGGL(k_predict_hash_count, dim3(grid), dim3(kBlock), 0, s, b.x, b.v, n, pd, g,
-                       b.keys, b.tickets, b.counts, b.big_count);
+hipError_t launch_predict_hash_count(const HashBufs& b, const StepArgs& a, hipStream_t s) {
+    const uint32_t grid = blocks_for(a.n > 0 ? a.n : 1, kBlock);
+    hipLaunchKernelGGL(k_predict_hash_count, dim3(grid), dim3(kBlock), 0, s, b.x, b.v, a, b.keys,
+                       b.tickets, b.counts, b.big_count);
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_gather(const HashBufs& b, uint32_t n, const PredictDev& pd,
-                                 hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const uint32_t grid = blocks_for(n, kBlock);
+hipError_t launch_scatter_gather(const HashBufs& b, const StepArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint32_t grid = blocks_for(a.n, kBlock);
     hipLaunchKernelGGL(k_scatter_ticket, dim3(grid), dim3(kBlock), 0, s, b.keys, b.tickets,
-                       b.cell_start, n, b.tmp_index, b.err);
+                       b.cell_start, a.n, b.tmp_index, b.err);
     hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp_index, b.keys,
-                       b.cell_start, b.x, b.v, n, pd, b.sorted_index, b.sorted, b.big_list,
+                       b.cell_start, b.x, b.v, a, b.sorted_index, b.sorted, b.big_list,
                        b.big_count, b.err);
     // Only cells above kSmallCell particles exist past this point; with
     // n <= kSmallCell there can be none.
-    if (n > kSmallCell)
+    if (a.n > kSmallCell)
         hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp_index, b.keys,
-                           b.cell_start, b.x, b.v, n, pd, b.big_list, b.big_count,
-                           b.sorted_index, b.sorted);
+                           b.cell_start, b.x, b.v, a, b.big_list, b.big_count, b.sorted_index,
+                           b.sorted);
     return hipGetLastError();
 }
 
 hipError_t launch_count_neighbors(const float4* sorted, const uint32_t* cell_start,
-                                  const uint32_t* sorted_index, uint32_t n, const GridDev& g,
-                                  float r2max, uint32_t* nbr, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_count_neighbors, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, sorted,
-                       cell_start, sorted_index, n, g, r2max, nbr);
+                                  const uint32_t* sorted_index, const StepArgs& a, uint32_t* nbr,
+                                  hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_neighbors, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, sorted,
+                       cell_start, sorted_index, a, nbr);
     return hipGetLastError();
 }
 

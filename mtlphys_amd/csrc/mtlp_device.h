@@ -31,15 +31,82 @@ __device__ __forceinline__ uint32_t key_of(const Cell3& c, const GridDev& g) {
     return (uint32_t)c.x + (uint32_t)c.y * g.dx + (uint32_t)c.z * g.dxy;
 }
 
+// Scene of particle (or sorted slot) i: the last s with p_off[s] <= i.
+// With one scene the result is the constant 0, so scene fields are read
+// with uniform (scalar) loads.
+__device__ __forceinline__ uint32_t scene_of(uint32_t i, const StepArgs& a) {
+    if (a.n_scenes == 1) return 0u;
+    uint32_t lo = 0, hi = a.n_scenes;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.p_off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The scene table read through the constant address space: with a
+// wave-uniform index these are scalar loads into SGPRs, like kernel
+// arguments.  The table is written only by the host between launches.
+typedef const __attribute__((address_space(4))) uint32_t* ConstWords;
+static_assert(sizeof(SceneDev) % 4 == 0, "scene constants are copied as dwords");
+
+__device__ __forceinline__ SceneDev load_scene_uniform(const SceneDev* table, uint32_t sid) {
+    union {
+        SceneDev s;
+        uint32_t w[sizeof(SceneDev) / 4];
+    } u;
+    const ConstWords src = (ConstWords)(table + sid);
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(SceneDev) / 4; ++k) u.w[k] = src[k];
+    return u.s;
+}
+
+// Calls body(constants of the scene of element i), the constants always
+// coming through scalar loads: each trip takes the scene of the first active
+// lane, runs the lanes of that scene, and leaves the others for the next trip
+// (one trip for a wave inside one scene; a wave straddling k scenes takes k).
+template <class Body>
+__device__ __forceinline__ void with_scene(uint32_t i, const StepArgs& a, Body&& body) {
+    const uint32_t sid = scene_of(i, a);
+    // The trip condition is the wave-uniform mask of lanes still to run, so
+    // the body stays inside the loop under a uniform scene index.  (Branching
+    // on sid == sid0, or leaving the loop after the body, lets the compiler
+    // put the per-lane sid back into the address.)
+    uint64_t todo = __ballot(true);
+    while (todo) {
+        const uint32_t sid0 = __builtin_amdgcn_readlane(sid, (int)__builtin_ctzll(todo));
+        const uint64_t same = __ballot(sid == sid0);
+        if ((same >> __lane_id()) & 1ull) {
+            const SceneDev sc = load_scene_uniform(a.scenes, sid0);
+            body(sc);
+        }
+        todo &= ~same;
+    }
+}
+
 // predictPositions (Constraints.metal:54-66): v' = v + g dt, x* = x + v' dt,
 // every operation rounded separately (no FMA) so the hash of x* is
 // reproducible on the CPU.
-__device__ __forceinline__ float4 predict_pos(float4 x, float4 v, const PredictDev& pd) {
+__device__ __forceinline__ float4 predict_pos(float4 x, float4 v, float gx, float gy, float gz,
+                                              float dt) {
 #pragma clang fp contract(off)
-    if (!pd.enable) return x;
-    const float vx = v.x + pd.gdtx, vy = v.y + pd.gdty, vz = v.z + pd.gdtz;
-    const float sx = vx * pd.dt, sy = vy * pd.dt, sz = vz * pd.dt;
+    const float gdtx = gx * dt, gdty = gy * dt, gdtz = gz * dt;
+    const float vx = v.x + gdtx, vy = v.y + gdty, vz = v.z + gdtz;
+    const float sx = vx * dt, sy = vy * dt, sz = vz * dt;
     return make_float4(x.x + sx, x.y + sy, x.z + sz, x.w);
+}
+
+// Position particle i is binned at: x* on the step path, x for a plain
+// spatial-hash build.
+__device__ __forceinline__ float4 binned_pos(const float4* __restrict__ x, const float4* __restrict__ v,
+                                             uint32_t i, const StepArgs& a) {
+    const float4 xi = x[i];
+    if (!a.predict) return xi;
+    const float4 vi = v[i];
+    float4 out;
+    with_scene(i, a, [&](const SceneDev& sc) { out = predict_pos(xi, vi, sc.gx, sc.gy, sc.gz, a.dt); });
+    return out;
 }
 
 // Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 inside each 16-lane

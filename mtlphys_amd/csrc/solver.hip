@@ -4,6 +4,10 @@
 // (:140-199) and scatterPositionsToOriginal + finalizeStep (:205-213,
 // :305-353) fused into one pass.
 //
+// Every kernel covers all scenes of a batch; a slot's scene constants come
+// from the scene table.  The stencil kernels are built for 128-, 256- and
+// 512-thread workgroups (the autotuner picks per batch).
+//
 // Layout: the sorted positions are float4; the density pass stores lambda_i
 // in .w of its own slot, so the delta pass reads (x_j, lambda_j) with one
 // 16-byte load per candidate.  The 27-cell stencil is walked as 9 rows of
@@ -66,13 +70,13 @@ struct Rows {
 };
 
 __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* __restrict__ cell_start,
-                                             const GridDev& g, uint32_t n) {
+                                             const GridDev& g, uint32_t cell_off, uint32_t n) {
     Rows r;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int z = st.z0 + k / 3, y = st.y0 + k % 3;
         const bool ok = z <= st.z1 && y <= st.y1;
-        const uint32_t row = ok ? (uint32_t)y * g.dx + (uint32_t)z * g.dxy : 0u;
+        const uint32_t row = cell_off + (ok ? (uint32_t)y * g.dx + (uint32_t)z * g.dxy : 0u);
         const uint32_t a = cell_start[row + (uint32_t)st.x0];
         const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
         r.a[k] = ok ? a : 0u;
@@ -143,21 +147,21 @@ __device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const
 }
 
 template <bool kOrig, bool kCount>
-__global__ __launch_bounds__(kBlock) void k_density_lambda(
-    float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
-    const uint32_t* __restrict__ sorted_index, uint32_t n, GridDev g, SolverDev sd,
-    float* __restrict__ lambda_orig, uint32_t* __restrict__ nbr_orig) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    if (s >= n) return;
+__device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& sc, float4* __restrict__ cur,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ sorted_index, uint32_t n,
+                                                    float* __restrict__ lambda_orig,
+                                                    uint32_t* __restrict__ nbr_orig) {
+    const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
-    const Stencil st = stencil_of(pi.x, pi.y, pi.z, g);
+    const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
     // sum of (h^2 - r^2)^3 in FP64; r's components are differences of
     // nearby FP32 values (exact by Sterbenz away from the origin), so the
     // density is accurate to FP64 rounding.
     double wsum = 0.0;
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
-    const Rows rows = stencil_rows(st, cell_start, g, n);
+    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
 #pragma unroll 1
     for (int k = 0; k < 9; ++k) {
         const uint32_t b = rows.b[k];
@@ -188,17 +192,30 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_apply_delta(
-    const float4* __restrict__ cur, const uint32_t* __restrict__ cell_start, uint32_t n, GridDev g,
-    SolverDev sd, float4* __restrict__ next) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+template <uint32_t kB, bool kOrig, bool kCount>
+__global__ __launch_bounds__(kB) void k_density_lambda(
+    float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
+    const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
+    uint32_t* __restrict__ nbr_orig) {
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t n = a.n;
     if (s >= n) return;
+    with_scene(s, a, [&](const SceneDev& sc) {
+        density_lambda_slot<kOrig, kCount>(s, sc, cur, cell_start, sorted_index, n, lambda_orig, nbr_orig);
+    });
+}
+
+__device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
+                                                 const float4* __restrict__ cur,
+                                                 const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                 float4* __restrict__ next) {
+    const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     const float lam_i = pi.w;
-    const Stencil st = stencil_of(pi.x, pi.y, pi.z, g);
+    const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
     const double lam_d = lam_i;
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    const Rows rows = stencil_rows(st, cell_start, g, n);
+    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
 #pragma unroll 1
     for (int k = 0; k < 9; ++k) {
         const uint32_t b = rows.b[k];
@@ -232,13 +249,25 @@ __global__ __launch_bounds__(kBlock) void k_apply_delta(
                           (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
 }
 
+template <uint32_t kB>
+__global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ cur,
+                                                   const uint32_t* __restrict__ cell_start,
+                                                   StepArgs a, float4* __restrict__ next) {
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t n = a.n;
+    if (s >= n) return;
+    with_scene(s, a, [&](const SceneDev& sc) { apply_delta_slot(s, sc, cur, cell_start, n, next); });
+}
+
 // Scatter back + finalize: slot s writes particle sorted_index[s].  The old
 // position (still in x) becomes prev; x and v get the step's result.
-__global__ __launch_bounds__(kBlock) void k_finalize(
-    const float4* __restrict__ fin, const uint32_t* __restrict__ sorted_index, uint32_t n,
-    FinalDev fd, float4* __restrict__ x, float4* __restrict__ v, float4* __restrict__ prev) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+__device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, float dt,
+                                              const float4* __restrict__ fin,
+                                              const uint32_t* __restrict__ sorted_index,
+                                              float4* __restrict__ x, float4* __restrict__ v,
+                                              float4* __restrict__ prev) {
+    const float inv_dt = 1.0f / dt;       // Engine.cpp:561
+    const float vmax = fd.vmax_h * inv_dt;  // Constraints.metal:347
     const uint32_t i = sorted_index[s];
     const float4 pf = fin[s];
     const float4 old = x[i];
@@ -246,9 +275,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(
     const float px = fminf(fmaxf(pf.x, fd.bminx), fd.bmaxx);
     const float py = fminf(fmaxf(pf.y, fd.bminy), fd.bmaxy);
     const float pz = fminf(fmaxf(pf.z, fd.bminz), fd.bmaxz);
-    float vx = (px - old.x) * fd.inv_dt * fd.damping;
-    float vy = (py - old.y) * fd.inv_dt * fd.damping;
-    float vz = (pz - old.z) * fd.inv_dt * fd.damping;
+    float vx = (px - old.x) * inv_dt * fd.damping;
+    float vy = (py - old.y) * inv_dt * fd.damping;
+    float vz = (pz - old.z) * inv_dt * fd.damping;
     const bool xmin = px <= fd.bminx + fd.weps, xmax = px >= fd.bmaxx - fd.weps;
     const bool ymin = py <= fd.bminy + fd.weps, ymax = py >= fd.bmaxy - fd.weps;
     const bool zmin = pz <= fd.bminz + fd.weps, zmax = pz >= fd.bmaxz - fd.weps;
@@ -267,8 +296,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(
         vz *= fd.wall_f;
     }
     const float vlen = sqrtf(vx * vx + vy * vy + vz * vz);
-    if (vlen > fd.vmax) {
-        const float f = fd.vmax / vlen;
+    if (vlen > vmax) {
+        const float f = vmax / vlen;
         vx *= f;
         vy *= f;
         vz *= f;
@@ -277,46 +306,70 @@ __global__ __launch_bounds__(kBlock) void k_finalize(
     v[i] = make_float4(vx, vy, vz, 0.0f);
 }
 
-inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+__global__ __launch_bounds__(kBlock) void k_finalize(
+    const float4* __restrict__ fin, const uint32_t* __restrict__ sorted_index, StepArgs a,
+    float4* __restrict__ x, float4* __restrict__ v, float4* __restrict__ prev) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= a.n) return;
+    with_scene(s, a, [&](const SceneDev& sc) { finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, v, prev); });
+}
+
+inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+template <uint32_t kB>
+void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a,
+                      float* lo, uint32_t* nb, hipStream_t s) {
+    const dim3 grid(blocks_for(a.n, kB)), block(kB);
+    if (lo && nb)
+        hipLaunchKernelGGL((k_density_lambda<kB, true, true>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+    else if (lo)
+        hipLaunchKernelGGL((k_density_lambda<kB, true, false>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+    else if (nb)
+        hipLaunchKernelGGL((k_density_lambda<kB, false, true>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+    else
+        hipLaunchKernelGGL((k_density_lambda<kB, false, false>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+}
 
 }  // namespace
 
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
-                                 const uint32_t* sorted_index, uint32_t n, const GridDev& g,
-                                 const SolverDev& sd, float* lambda_orig, uint32_t* nbr_orig,
+                                 const uint32_t* sorted_index, const StepArgs& a,
+                                 float* lambda_orig, uint32_t* nbr_orig, uint32_t block,
                                  hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    const dim3 grid(blocks_for(n)), block(kBlock);
-    if (lambda_orig && nbr_orig)
-        hipLaunchKernelGGL((k_density_lambda<true, true>), grid, block, 0, s, cur, cell_start,
-                           sorted_index, n, g, sd, lambda_orig, nbr_orig);
-    else if (lambda_orig)
-        hipLaunchKernelGGL((k_density_lambda<true, false>), grid, block, 0, s, cur, cell_start,
-                           sorted_index, n, g, sd, lambda_orig, nbr_orig);
-    else if (nbr_orig)
-        hipLaunchKernelGGL((k_density_lambda<false, true>), grid, block, 0, s, cur, cell_start,
-                           sorted_index, n, g, sd, lambda_orig, nbr_orig);
-    else
-        hipLaunchKernelGGL((k_density_lambda<false, false>), grid, block, 0, s, cur, cell_start,
-                           sorted_index, n, g, sd, lambda_orig, nbr_orig);
+    if (a.n == 0) return hipSuccess;
+    switch (block) {
+        case 128: density_lambda_b<128>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
+        case 512: density_lambda_b<512>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
+        default: density_lambda_b<256>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, uint32_t n,
-                              const GridDev& g, const SolverDev& sd, float4* next,
-                              hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_apply_delta, dim3(blocks_for(n)), dim3(kBlock), 0, s, cur, cell_start, n,
-                       g, sd, next);
+hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
+                              float4* next, uint32_t block, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    switch (block) {
+        case 128:
+            hipLaunchKernelGGL(k_apply_delta<128>, dim3(blocks_for(a.n, 128)), dim3(128), 0, s, cur,
+                               cell_start, a, next);
+            break;
+        case 512:
+            hipLaunchKernelGGL(k_apply_delta<512>, dim3(blocks_for(a.n, 512)), dim3(512), 0, s, cur,
+                               cell_start, a, next);
+            break;
+        default:
+            hipLaunchKernelGGL(k_apply_delta<256>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, cur,
+                               cell_start, a, next);
+            break;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index, uint32_t n,
-                           const FinalDev& fd, float4* x, float4* v, float4* prev,
-                           hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(n)), dim3(kBlock), 0, s, final_sorted,
-                       sorted_index, n, fd, x, v, prev);
+hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
+                           const StepArgs& a, float4* x, float4* v, float4* prev, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, final_sorted,
+                       sorted_index, a, x, v, prev);
     return hipGetLastError();
 }
 

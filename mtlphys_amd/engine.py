@@ -223,3 +223,74 @@ class Engine:
         f = ctypes.c_uint32()
         C.check(C.lib().mtlp_error_flags(self._h, ctypes.byref(f)), "mtlp_error_flags")
         return f.value
+
+
+class Batch(Engine):
+    """Many independent scenes stepped together, one kernel launch per stage.
+
+    Scene s owns particles [particle_offsets[s], particle_offsets[s+1]) of
+    every per-particle buffer, in original and in sorted order.  Each scene
+    has its own parameters (bounds/grid, rest density, gravity, ...); all
+    share the iteration count."""
+
+    def __init__(self, counts, params, device=0):
+        self._h = None
+        counts = [int(c) for c in counts]
+        if len(params) != len(counts):
+            raise ValueError("one parameter set per scene")
+        arr_c = (ctypes.c_uint32 * len(counts))(*counts)
+        arr_p = (C.Params * len(params))(*params)
+        h = ctypes.c_void_p()
+        C.check(C.lib().mtlp_batch_create(int(device), len(counts), arr_c, arr_p, ctypes.byref(h)),
+                "mtlp_batch_create")
+        self._h = h
+        self._scene = C.SCENE_EMPTY
+        self.counts = counts
+        self.scene_params = [p.copy() for p in params]
+
+    def seed(self, scene, seed=0):
+        """Seeds scene s with generate_scene(scene, counts[s], seed + s, params[s])."""
+        C.check(C.lib().mtlp_batch_seed(self._h, int(scene), int(seed)), "mtlp_batch_seed")
+        self._scene = scene
+
+    def offsets(self):
+        ns = ctypes.c_uint32()
+        C.check(C.lib().mtlp_batch_info(self._h, ctypes.byref(ns), None, None), "mtlp_batch_info")
+        po = (ctypes.c_uint32 * (ns.value + 1))()
+        co = (ctypes.c_uint32 * (ns.value + 1))()
+        C.check(C.lib().mtlp_batch_info(self._h, None, po, co), "mtlp_batch_info")
+        return np.array(po, np.uint32), np.array(co, np.uint32)
+
+    def set_scene_params(self, scene, p):
+        C.check(C.lib().mtlp_batch_set_params(self._h, int(scene), ctypes.byref(p)),
+                "mtlp_batch_set_params")
+        self.scene_params[scene] = p.copy()
+
+
+def _engine_extras():
+    def step_n(self, dt, n_steps, stream=None):
+        C.check(C.lib().mtlp_step_n(self._h, _stream_handle(stream), ctypes.c_float(dt), int(n_steps)),
+                "mtlp_step_n")
+
+    def set_graph(self, on=True):
+        """Capture each step into a HIP graph (once per dt) and replay it."""
+        C.check(C.lib().mtlp_set_graph(self._h, 1 if on else 0), "mtlp_set_graph")
+
+    def autotune(self, dt, cache_path=None, stream=None):
+        """Pick the stencil workgroup size by timing; returns it."""
+        b = ctypes.c_uint32()
+        C.check(C.lib().mtlp_autotune(self._h, _stream_handle(stream), ctypes.c_float(dt),
+                                      None if cache_path is None else str(cache_path).encode(),
+                                      ctypes.byref(b)), "mtlp_autotune")
+        return b.value
+
+    def set_stencil_block(self, block):
+        C.check(C.lib().mtlp_set_stencil_block(self._h, int(block)), "mtlp_set_stencil_block")
+
+    Engine.step_n = step_n
+    Engine.set_graph = set_graph
+    Engine.autotune = autotune
+    Engine.set_stencil_block = set_stencil_block
+
+
+_engine_extras()
