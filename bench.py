@@ -143,6 +143,52 @@ def batched_leg(steps, warmup, stream, torch):
                     "cube drop with the same total particle count"}
 
 
+def adjoint_leg(torch, reps=3):
+    """The differentiable workload (65,536 particles, h = 0.04, 3 iterations,
+    40 steps, checkpoints every 8): forward rollout + adjoint against 40
+    plain steps from the same state, median of `reps`."""
+    from mtlphys_amd import workloads as W
+    import mtlphys_amd as mtlp
+    p, x, v, target = W.differentiable()
+    n, steps, every = len(x), W.ADJ_STEPS, W.ADJ_CKPT
+    eng = mtlp.Engine(0, p)
+    eng.reset(n, mtlp.capi.SCENE_EMPTY)
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream()
+    xbar = torch.zeros((n, 4), device=dev)
+    xbar[:, 0] = 1.0 / n  # the centroid term's shape: any fixed adjoint input costs the same
+    xb0, vb0, gb = (torch.empty((n, 4), device=dev), torch.empty((n, 4), device=dev), torch.empty(3, device=dev))
+    t_plain, t_fwd, t_adj = [], [], []
+    for rep in range(reps + 1):  # rep 0 warms up (allocations, code load)
+        eng.upload_state(x, v)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.step_n(DT, steps, stream)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        eng.upload_state(x, v)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        eng.rollout(DT, steps, every, stream)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        eng.adjoint_rollout(DT, steps, every, xbar, None, xb0, vb0, gb, stream)
+        torch.cuda.synchronize()
+        t4 = time.perf_counter()
+        if rep:
+            t_plain.append(t1 - t0)
+            t_fwd.append(t3 - t2)
+            t_adj.append(t4 - t3)
+    flags = eng.error_flags()
+    finite = bool(torch.isfinite(vb0).all().item() and torch.isfinite(gb).all().item())
+    eng.close()
+    tp, tf, ta = (statistics.median(t) for t in (t_plain, t_fwd, t_adj))
+    return {"workload": "differentiable_64k", "particles": n, "steps": steps, "ckpt_every": every,
+            "iterations": p.iterations, "plain_ms": round(tp * 1e3, 3), "rollout_ms": round(tf * 1e3, 3),
+            "adjoint_ms": round(ta * 1e3, 3), "fwd_plus_adjoint_over_plain": round((tf + ta) / tp, 3),
+            "finite": finite, "error_flags": flags}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +196,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (reference kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -250,6 +297,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "batched": None if args.no_batched else batched_leg(args.steps, args.warmup, stream, torch),
+        "adjoint": None if args.no_adjoint else adjoint_leg(torch),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(x, v, params),
     }
     print(json.dumps(out), flush=True)

@@ -140,6 +140,11 @@ mtlp_status mtlp_get_params(const mtlp_engine* e, mtlp_params* out);
  * velocity buffers (tests/test_pbf.cpp:197-200).  Either pointer may be
  * NULL.  Synchronous. */
 mtlp_status mtlp_upload_state(mtlp_engine* e, const float* x4, const float* v4, uint32_t n);
+/* The same from / to device memory (n float4 each, original order),
+ * asynchronous on `stream`; either pointer may be NULL.  For callers whose
+ * state lives in their own device tensors. */
+mtlp_status mtlp_set_state_device(mtlp_engine* e, mtlp_stream stream, const float* d_x4, const float* d_v4);
+mtlp_status mtlp_get_state_device(mtlp_engine* e, mtlp_stream stream, float* d_x4, float* d_v4);
 
 /* Engine::step(cmd, dt) (Engine.cpp:685-811): predict, spatial hash,
  * iterations x (density/lambda, delta), scatter back, finalize.  dt <= 0 is
@@ -177,8 +182,8 @@ mtlp_status mtlp_set_profiling(mtlp_engine* e, int enable);
 mtlp_status mtlp_stage_times(mtlp_engine* e, float* ms, const char** names,
                              uint32_t capacity, uint32_t* count);
 /* Device error flags raised by kernels since the last call (bit 0: scan
- * look-back gave up waiting; bit 1: cell offsets inconsistent);
- * synchronises and clears them. */
+ * look-back gave up waiting; bit 1: cell offsets inconsistent; bit 2: the
+ * rollout adjoint left the FP32 range); synchronises and clears them. */
 mtlp_status mtlp_error_flags(mtlp_engine* e, uint32_t* flags);
 
 /* ---- Batched ensembles (no reference counterpart: the reference steps one
@@ -214,6 +219,60 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
                           uint32_t* block);
 /* Workgroup size of the stencil kernels (128, 256 or 512). */
 mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
+
+/* ---- differentiable rollout (single-scene engines) --------------------
+ * The reference has no adjoint; these entries have no counterpart in it.
+ * The adjoint differentiates mtlp_step as computed, with the neighbour
+ * structure held fixed (binning and candidate sets are piecewise
+ * constant), the box clamps as pass-through inside the box and 0 outside,
+ * the 0.3 h step cap and the h/dt speed cap through their normalisation
+ * Jacobians, and the wall responses as piecewise-constant multipliers.
+ *
+ * Device arrays of particles are float4 (x, y, z, w) in original order;
+ * the w of every gradient output is 0 and the w of gradient inputs is
+ * ignored. */
+
+/* Steps the engine `steps` times (as mtlp_step) and keeps (x, v) before
+ * steps 0, k, 2k, ... (k = ckpt_every) in engine memory for
+ * mtlp_adjoint_rollout.  The engine ends at step `steps`. */
+mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps,
+                         uint32_t ckpt_every);
+/* Adjoint of the last mtlp_rollout (same dt, steps and ckpt_every): from
+ * dL/dx_T and dL/dv_T (NULL = 0) to dL/dx_0, dL/dv_0 and dL/dgravity (3
+ * floats), each written only if non-NULL.  Each segment of ckpt_every
+ * steps is recomputed from its checkpoint with a tape, then walked
+ * backwards.  The engine's state (x_T, v_T) is not changed. */
+mtlp_status mtlp_adjoint_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps,
+                                 uint32_t ckpt_every, const float* d_xbar_T, const float* d_vbar_T,
+                                 float* d_xbar_0, float* d_vbar_0, float* d_gbar);
+/* Density rho_i = W(0) + sum_j W(|x_i - x_j|) (Constraints.metal:100-115)
+ * at positions d_x4 (n float4), with the spatial hash rebuilt from d_x4 in
+ * the engine's hash buffers; d_rho receives n floats. */
+mtlp_status mtlp_density(mtlp_engine* e, mtlp_stream stream, const float* d_x4, float* d_rho);
+/* dL/dx at d_x4 from dL/drho (n floats) for mtlp_density. */
+mtlp_status mtlp_density_vjp(mtlp_engine* e, mtlp_stream stream, const float* d_x4, const float* d_rhobar,
+                             float* d_xbar);
+
+/* Tape of the taped forward, for inspection.  mtlp_tape_record runs
+ * `steps` taped steps from the engine's current state without changing
+ * it; mtlp_tape_buffer returns a borrowed device pointer into the tape of
+ * step `step` (< steps): */
+typedef enum mtlp_tape_kind {
+    MTLP_TAPE_POSITIONS = 0,     /* float4, state at the step's start (original order) */
+    MTLP_TAPE_VELOCITIES = 1,    /* float4 */
+    MTLP_TAPE_SORTED_INDEX = 2,  /* u32 */
+    MTLP_TAPE_CELL_START = 3,    /* u32, total_cells + 1 */
+    MTLP_TAPE_ITERATE = 4,       /* float4 sorted, iteration 0..I; .w = lambda of that
+                                    iteration for iteration < I */
+    MTLP_TAPE_GRAD_SUMS = 5,     /* float4 sorted (sum_j grad W, D), iteration < I */
+    MTLP_TAPE_DELTA = 6          /* float4 sorted, R before the 0.3 h cap, iteration < I */
+} mtlp_tape_kind;
+mtlp_status mtlp_tape_record(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps);
+mtlp_status mtlp_tape_buffer(mtlp_engine* e, mtlp_tape_kind kind, uint32_t step, uint32_t iteration,
+                             void** d_ptr, size_t* bytes);
+/* Synchronous copy of a tape array to host memory (bytes <= its size). */
+mtlp_status mtlp_tape_read(mtlp_engine* e, mtlp_tape_kind kind, uint32_t step, uint32_t iteration,
+                           void* host_dst, size_t bytes);
 
 #ifdef __cplusplus
 }

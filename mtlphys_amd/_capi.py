@@ -30,6 +30,14 @@ BUF_NEIGHBOR_COUNTS = 6
 BUF_LAMBDAS = 7
 BUF_SORTED_POSITIONS = 8
 
+TAPE_POSITIONS = 0
+TAPE_VELOCITIES = 1
+TAPE_SORTED_INDEX = 2
+TAPE_CELL_START = 3
+TAPE_ITERATE = 4
+TAPE_GRAD_SUMS = 5
+TAPE_DELTA = 6
+
 MAX_SCAN_LENGTH = 0x7FFFFFFF
 MAX_PARTICLES = 0x40000000
 
@@ -110,6 +118,15 @@ SIGNATURES = {
     "mtlp_set_graph": (_I, [_P, _I]),
     "mtlp_autotune": (_I, [_P, _P, _F, ctypes.c_char_p, ctypes.POINTER(_U32)]),
     "mtlp_set_stencil_block": (_I, [_P, _U32]),
+    "mtlp_set_state_device": (_I, [_P, _P, _P, _P]),
+    "mtlp_get_state_device": (_I, [_P, _P, _P, _P]),
+    "mtlp_rollout": (_I, [_P, _P, _F, _U32, _U32]),
+    "mtlp_adjoint_rollout": (_I, [_P, _P, _F, _U32, _U32, _P, _P, _P, _P, _P]),
+    "mtlp_density": (_I, [_P, _P, _P, _P]),
+    "mtlp_density_vjp": (_I, [_P, _P, _P, _P, _P]),
+    "mtlp_tape_record": (_I, [_P, _P, _F, _U32]),
+    "mtlp_tape_buffer": (_I, [_P, _I, _U32, _U32, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
+    "mtlp_tape_read": (_I, [_P, _I, _U32, _U32, _P, ctypes.c_size_t]),
 }
 
 _lib = None

@@ -17,8 +17,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmtlp.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-SOURCES = ["engine.cpp", "scenes.cpp", "scan.hip", "hash.hip", "solver.hip"]
-HEADERS = ["mtlp_kernels.h", "mtlp_device.h", "mtlp_host.h"]
+SOURCES = ["engine.cpp", "adjoint.cpp", "scenes.cpp", "scan.hip", "hash.hip", "solver.hip", "adjoint.hip"]
+HEADERS = ["mtlp_kernels.h", "mtlp_device.h", "mtlp_stencil.h", "mtlp_host.h", "engine_impl.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -33,7 +33,7 @@ FLAGS = [
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(BUILD, src.replace(".", "_") + ".o")
     cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -19,52 +19,20 @@
 #include <string>
 #include <vector>
 
-#include "mtlp_host.h"
-#include "mtlp_kernels.h"
+#include "engine_impl.h"
 
 using namespace mtlp;
 
-namespace {
+namespace mtlp {
 
+namespace {
 thread_local std::string g_last_error;
+}  // namespace
 
 mtlp_status fail(mtlp_status s, const std::string& msg) {
     g_last_error = msg;
     return s;
 }
-
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        const hipError_t e_ = (expr);                                                     \
-        if (e_ != hipSuccess)                                                             \
-            return fail(MTLP_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
-    } while (0)
-
-#define MTLP_TRY(expr)                          \
-    do {                                        \
-        const mtlp_status st_ = (expr);         \
-        if (st_ != MTLP_OK) return st_;         \
-    } while (0)
-
-struct DeviceGuard {
-    int prev = -1;
-    bool ok = false;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        ok = hipSetDevice(dev) == hipSuccess;
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
-constexpr uint32_t kPad = 256;  // particle buffers padded like Engine.cpp:383
-constexpr uint32_t kMaxIterations = 256;
-constexpr uint32_t kMaxScorrN = 64;
-constexpr uint32_t kEpochLimit = (1u << 30) - 1u;
-constexpr uint32_t kMaxScenes = 1u << 20;
-
-uint32_t round_up(uint32_t n, uint32_t m) { return ((n + m - 1) / m) * m; }
 
 mtlp_status validate_params(const mtlp_params* p, uint32_t dims[3]) {
     if (!p) return fail(MTLP_E_INVALID, "params is NULL");
@@ -156,145 +124,9 @@ FinalDev make_final(const mtlp_params& p) {
     return f;
 }
 
-template <class T>
-hipError_t dev_alloc(T** p, size_t count) {
-    *p = nullptr;
-    return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (count ? count : 1));
-}
-
-template <class T>
-void dev_free(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-}  // namespace
-
-struct mtlp_engine {
-    int device = 0;
-    // scenes (host)
-    std::vector<mtlp_params> scene_params;
-    std::vector<std::array<uint32_t, 3>> scene_dims;
-    std::vector<uint32_t> scene_counts;
-    std::vector<uint32_t> p_off;     // n_scenes + 1
-    std::vector<uint32_t> cell_off;  // n_scenes + 1
-    bool batch = false;
-    uint32_t cells = 0;
-    uint32_t n = 0;
-    uint32_t n_alloc = 0;
-    uint32_t scene = MTLP_SCENE_CUBE_DROP;
-    bool allocated = false;
-    uint32_t stencil_block = 256;
-
-    // scene table (device)
-    SceneDev* d_scenes = nullptr;
-    uint32_t* d_p_off = nullptr;
-    // particle buffers (original order)
-    float4* x = nullptr;
-    float4* v = nullptr;
-    float4* prev = nullptr;
-    uint32_t* keys = nullptr;
-    uint32_t* tickets = nullptr;
-    uint32_t* nbr = nullptr;
-    float* lambdas = nullptr;
-    // particle buffers (sorted order)
-    uint32_t* tmp_index = nullptr;
-    uint32_t* sorted_index = nullptr;
-    float4* sorted[2] = {nullptr, nullptr};
-    uint32_t* big_list = nullptr;
-    int final_buf = 0;
-    // cell buffers
-    uint32_t* counts = nullptr;
-    uint32_t* cell_start = nullptr;
-    uint64_t* scan_state = nullptr;
-    uint32_t scan_epoch = 0;
-    // misc device words: [0] big-cell count, [1] scan ticket, [2] error flags
-    uint32_t* misc = nullptr;
-
-    // profiling
-    bool profiling = false;
-    std::vector<hipEvent_t> events;
-    std::vector<const char*> stage_names;
-    uint32_t stages_recorded = 0;
-
-    // graph replay
-    bool use_graph = false;
-    bool capturing = false;
-    hipStream_t capture_stream = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    float graph_dt = 0.0f;
-
-    uint32_t iterations() const { return scene_params.empty() ? 0u : scene_params[0].iterations; }
-
-    void drop_graph() {
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        graph_exec = nullptr;
-    }
-    void free_particles() {
-        dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
-        dev_free(nbr); dev_free(lambdas); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list);
-        n_alloc = 0;
-    }
-    void free_cells() {
-        dev_free(counts); dev_free(cell_start); dev_free(scan_state);
-        cells = 0;
-    }
-    ~mtlp_engine() {
-        DeviceGuard g(device);
-        drop_graph();
-        if (capture_stream) (void)hipStreamDestroy(capture_stream);
-        free_particles();
-        free_cells();
-        dev_free(misc);
-        dev_free(d_scenes);
-        dev_free(d_p_off);
-        for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
-    }
-
-    HashBufs hash_bufs() {
-        HashBufs b{};
-        b.x = x; b.v = v; b.keys = keys; b.tickets = tickets; b.counts = counts;
-        b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
-        b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
-        return b;
-    }
-
-    StepArgs step_args(float dt, bool predict) const {
-        StepArgs a{};
-        a.scenes = d_scenes;
-        a.p_off = d_p_off;
-        a.n_scenes = (uint32_t)scene_params.size();
-        a.n = n;
-        a.dt = dt;
-        a.predict = predict ? 1u : 0u;
-        return a;
-    }
-};
+}  // namespace mtlp
 
 namespace {
-
-// Records a HIP event after each kernel when profiling is on.
-struct StageClock {
-    mtlp_engine* e;
-    hipStream_t s;
-    bool on;
-    uint32_t k = 0;
-    StageClock(mtlp_engine* eng, hipStream_t stream)
-        : e(eng), s(stream), on(eng->profiling && !eng->capturing) {
-        if (!on) return;
-        e->stage_names.clear();
-        e->stages_recorded = 0;
-        (void)hipEventRecord(e->events[0], s);
-    }
-    void mark(const char* name) {
-        if (!on || k + 1 >= e->events.size()) return;
-        ++k;
-        (void)hipEventRecord(e->events[k], s);
-        e->stage_names.push_back(name);
-        e->stages_recorded = k;
-    }
-};
 
 // Scene layout from per-scene params and counts: grids, cell and particle
 // offsets.  Validates that every scene shares the iteration count.
@@ -396,6 +228,7 @@ mtlp_status allocate_all(mtlp_engine* e) {
     DeviceGuard g(e->device);
     HIP_TRY(hipDeviceSynchronize());
     e->allocated = false;
+    e->adj.release();
     MTLP_TRY(layout_scenes(e));
     MTLP_TRY(alloc_particles(e, e->p_off.back()));
     MTLP_TRY(alloc_cells(e, e->cell_off.back()));
@@ -425,10 +258,12 @@ mtlp_status seed_scenes(mtlp_engine* e, uint32_t scene, uint64_t seed) {
     return MTLP_OK;
 }
 
-// Spatial hash into sorted[0]: predict (optional) + hash + count, scan,
-// stable scatter + gather.  Engine.cpp:601-667 and :715-726.
-mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, StageClock& clk) {
-    HashBufs b = e->hash_bufs();
+}  // namespace
+
+namespace mtlp {
+
+// Engine.cpp:601-667 and :715-726.
+mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, StageClock& clk, const HashBufs& b) {
     HIP_TRY(launch_predict_hash_count(b, a, s));
     clk.mark("predict_hash_count");
     uint32_t epoch;
@@ -444,7 +279,7 @@ mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, Stage
         }
         epoch = ++e->scan_epoch;
     }
-    HIP_TRY(launch_exclusive_scan(e->counts, e->cell_start, e->cells + 1, true, e->scan_state,
+    HIP_TRY(launch_exclusive_scan(b.counts, b.cell_start, e->cells + 1, true, e->scan_state,
                                   e->misc + 1, epoch, e->misc + 2, s));
     clk.mark("exclusive_scan");
     HIP_TRY(launch_scatter_gather(b, a, s));
@@ -452,11 +287,21 @@ mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, Stage
     return MTLP_OK;
 }
 
+mtlp_status check_engine(const mtlp_engine* e) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (!e->allocated) return fail(MTLP_E_INVALID, "engine has no buffers: call mtlp_reset first");
+    return MTLP_OK;
+}
+
+}  // namespace mtlp
+
+namespace {
+
 // Engine::step (Engine.cpp:685-811) for every scene.
 mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     StageClock clk(e, s);
     const StepArgs a = e->step_args(dt, true);
-    MTLP_TRY(enqueue_hash(e, s, a, clk));
+    MTLP_TRY(enqueue_hash(e, s, a, clk, e->hash_bufs()));
     const uint32_t iters = e->iterations();
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
@@ -498,12 +343,6 @@ mtlp_status capture_graph(mtlp_engine* e, float dt) {
     (void)hipGraphDestroy(graph);
     HIP_TRY(inst);
     e->graph_dt = dt;
-    return MTLP_OK;
-}
-
-mtlp_status check_engine(const mtlp_engine* e) {
-    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
-    if (!e->allocated) return fail(MTLP_E_INVALID, "engine has no buffers: call mtlp_reset first");
     return MTLP_OK;
 }
 
@@ -652,6 +491,7 @@ mtlp_status mtlp_batch_set_params(mtlp_engine* e, uint32_t scene, const mtlp_par
     MTLP_TRY(validate_params(p, dims));
     const mtlp_params old = e->scene_params[scene];
     e->scene_params[scene] = *p;
+    e->adj.recorded = false;  // checkpoints were taken under the old parameters
     if (!e->allocated) return MTLP_OK;
     const auto& d = e->scene_dims[scene];
     if (dims[0] != d[0] || dims[1] != d[1] || dims[2] != d[2]) {
@@ -698,6 +538,28 @@ mtlp_status mtlp_upload_state(mtlp_engine* e, const float* x4, const float* v4, 
     HIP_TRY(hipDeviceSynchronize());
     if (x4) HIP_TRY(hipMemcpy(e->x, x4, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
     if (v4) HIP_TRY(hipMemcpy(e->v, v4, sizeof(float) * 4 * (size_t)n, hipMemcpyHostToDevice));
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_set_state_device(mtlp_engine* e, mtlp_stream stream, const float* d_x4, const float* d_v4) {
+    MTLP_TRY(check_engine(e));
+    if (e->n == 0) return MTLP_OK;
+    DeviceGuard g(e->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t bytes = sizeof(float4) * e->n;
+    if (d_x4) HIP_TRY(hipMemcpyAsync(e->x, d_x4, bytes, hipMemcpyDeviceToDevice, s));
+    if (d_v4) HIP_TRY(hipMemcpyAsync(e->v, d_v4, bytes, hipMemcpyDeviceToDevice, s));
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_get_state_device(mtlp_engine* e, mtlp_stream stream, float* d_x4, float* d_v4) {
+    MTLP_TRY(check_engine(e));
+    if (e->n == 0) return MTLP_OK;
+    DeviceGuard g(e->device);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t bytes = sizeof(float4) * e->n;
+    if (d_x4) HIP_TRY(hipMemcpyAsync(d_x4, e->x, bytes, hipMemcpyDeviceToDevice, s));
+    if (d_v4) HIP_TRY(hipMemcpyAsync(d_v4, e->v, bytes, hipMemcpyDeviceToDevice, s));
     return MTLP_OK;
 }
 
@@ -833,7 +695,7 @@ mtlp_status mtlp_build_spatial_hash(mtlp_engine* e, mtlp_stream stream) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     StageClock clk(e, s);
     const StepArgs a = e->step_args(0.0f, false);
-    MTLP_TRY(enqueue_hash(e, s, a, clk));
+    MTLP_TRY(enqueue_hash(e, s, a, clk, e->hash_bufs()));
     HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
     clk.mark("count_neighbors");
     e->final_buf = 0;

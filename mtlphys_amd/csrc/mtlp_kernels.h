@@ -1,5 +1,6 @@
-// mtlp_kernels.h -- launch interface between the C++ engine (engine.cpp) and
-// the gfx950 kernels (hash.hip, scan.hip, solver.hip).
+// mtlp_kernels.h -- launch interface between the C++ engine (engine.cpp,
+// adjoint.cpp) and the gfx950 kernels (hash.hip, scan.hip, solver.hip,
+// adjoint.hip).
 //
 // Every kernel works on a batch of independent scenes laid end to end: scene
 // s owns particles [p_off[s], p_off[s+1]) in original order AND sorted slots
@@ -29,6 +30,7 @@ constexpr uint32_t kBigGrid = 128;        // workgroups of the big-cell kernel
 // Error flag bits (device word, read by mtlp_error_flags).
 constexpr uint32_t kErrScanLookback = 1u;  // a scan tile gave up waiting
 constexpr uint32_t kErrHashRange = 2u;     // cell offsets inconsistent with n
+constexpr uint32_t kErrAdjointRange = 4u;  // the adjoint left the FP32 range (inf / NaN)
 
 // Grid of the spatial hash (Shared.h:40-49).
 struct GridDev {
@@ -124,5 +126,60 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
                               float4* next, uint32_t block, hipStream_t s);
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
                            const StepArgs& a, float4* x, float4* v, float4* prev, hipStream_t s);
+
+// ---- adjoint (adjoint.hip) ---------------------------------------------
+// What the adjoint of one step reads from its taped forward: the hash, the
+// iterates p^0 .. p^I (sorted float4; p^k.w = lambda^k for k < I), per
+// iteration the density pass's gradient sums (sum_j grad W, D) and the
+// delta pass's R (before the 0.3 h cap), and the step's input positions.
+struct StepTape {
+    const uint32_t* sorted_index;
+    const uint32_t* cell_start;
+    const float4* x_in;      // original order
+    float4* iter;            // I + 1 arrays of `stride` float4
+    float4* grad;            // I arrays
+    float4* dlt;             // I arrays
+    size_t stride;
+    float4* iterate(uint32_t k) const { return iter + k * stride; }
+    float4* grad_sums(uint32_t k) const { return grad + k * stride; }
+    float4* delta(uint32_t k) const { return dlt + k * stride; }
+};
+
+// Work buffers of the backward pass (sorted order unless noted).  prem and
+// lrem receive the remainder atomics and must be zero between steps (the
+// pointwise stages that consume them clear them).
+struct AdjointBufs {
+    float4* pacc;   // adjoint of the current iterate, own part
+    float4* prem;   // its remainder part
+    float4* rbar;   // adjoint of R
+    float4* db;     // (2 qbar g, qbar)
+    float* lbar;    // adjoint of lambda, own part
+    float* lrem;    // its remainder part
+    double* gdev;   // 6 per slot: deviation terms of the gravity recursion
+    double* gpart;  // 6 per block of kBlock slots
+    double* gstate; // 9: sum xbar, sum vbar, the gravity gradient so far
+    uint32_t* err;  // error flag word
+};
+
+hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const StepArgs& a,
+                                      float4* aux, uint32_t block, hipStream_t s);
+hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
+                                   float4* next, float4* rr, uint32_t block, hipStream_t s);
+hipError_t launch_finalize_tape(const float4* fin, const uint32_t* sorted_index, const StepArgs& a,
+                                const float4* x_in, float4* x_out, float4* v_out, hipStream_t s);
+// Starts the gravity recursion from the adjoint inputs (xbar_T, vbar_T).
+hipError_t launch_gravity_init(const AdjointBufs& b, const float4* xbar_T, const float4* vbar_T, uint32_t n,
+                               float* gout, hipStream_t s);
+// (xbar_out, vbar_out) of one step -> (xbar_in, vbar_in); the gravity
+// gradient so far goes to b.gstate and, as float, to gout[3].
+hipError_t launch_step_vjp(const StepTape& t, const AdjointBufs& b, const StepArgs& a, uint32_t iterations,
+                           float damping, const float4* xbar_out, const float4* vbar_out, float4* xbar_in,
+                           float4* vbar_in, float* gout, uint32_t block, hipStream_t s);
+// Density at the positions of a fresh spatial hash (sorted, cell_start,
+// sorted_index), original order, and its adjoint (sorted.w is overwritten).
+hipError_t launch_density_rho(const float4* sorted, const uint32_t* cell_start, const uint32_t* sorted_index,
+                              const StepArgs& a, float* rho, hipStream_t s);
+hipError_t launch_density_rho_vjp(float4* sorted, const uint32_t* cell_start, const uint32_t* sorted_index,
+                                  const StepArgs& a, const float* rhobar, float4* xbar, hipStream_t s);
 
 }  // namespace mtlp

@@ -2,195 +2,16 @@
 //
 // densityLambdaSorted (Constraints.metal:82-131), applyDeltaSorted
 // (:140-199) and scatterPositionsToOriginal + finalizeStep (:205-213,
-// :305-353) fused into one pass.
+// :305-353) fused into one pass.  The per-slot code is in mtlp_stencil.h.
 //
 // Every kernel covers all scenes of a batch; a slot's scene constants come
 // from the scene table.  The stencil kernels are built for 128-, 256- and
 // 512-thread workgroups (the autotuner picks per batch).
-//
-// Layout: the sorted positions are float4; the density pass stores lambda_i
-// in .w of its own slot, so the delta pass reads (x_j, lambda_j) with one
-// 16-byte load per candidate.  The 27-cell stencil is walked as 9 rows of
-// three x-adjacent cells: their slots are contiguous in cell order, so each
-// row is one slot range [cell_start[x0], cell_start[x1 + 1]), visited in the
-// reference's z, y, x, slot order.
-#include "mtlp_device.h"
+#include "mtlp_stencil.h"
 
 namespace mtlp {
 
 namespace {
-
-struct Stencil {
-    int x0, x1, y0, y1, z0, z1;
-};
-
-__device__ __forceinline__ Stencil stencil_of(float px, float py, float pz, const GridDev& g) {
-    const Cell3 c = cell_of(px, py, pz, g);
-    return Stencil{max(c.x - 1, 0), min(c.x + 1, (int)g.dx - 1), max(c.y - 1, 0),
-                   min(c.y + 1, (int)g.dy - 1), max(c.z - 1, 0), min(c.z + 1, (int)g.dz - 1)};
-}
-
-// |r|^2 as ((rx rx + ry ry) + rz rz) without FMA: the neighbour test
-// r2 < h^2 then decides exactly as the oracle's FP32 arithmetic does.
-__device__ __forceinline__ float dist2(float rx, float ry, float rz) {
-#pragma clang fp contract(off)
-    return rx * rx + ry * ry + rz * rz;
-}
-
-// Candidates are filtered before any SPH arithmetic: bit k of the result is
-// set when slot c0 + k (k < c1 - c0 <= 32) is a neighbour of s, i.e.
-// |p_s - p_j|^2 < h^2 and j != s.  The expensive terms then run only on set
-// bits, so lanes of a wave stay busy on real neighbours.  Loads go out eight
-// at a time (addresses clamped into [c0, c1)) before any is consumed.
-__device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur, uint32_t c0,
-                                                  uint32_t c1, uint32_t s, float4 pi, float h2) {
-    uint32_t mask = 0;
-    const uint32_t last = c1 - 1;
-#pragma unroll
-    for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
-        if (c0 + k0 >= c1) break;
-        float4 q[8];
-#pragma unroll
-        for (uint32_t u = 0; u < 8; ++u) q[u] = cur[min(c0 + k0 + u, last)];
-#pragma unroll
-        for (uint32_t u = 0; u < 8; ++u) {
-            const uint32_t j = c0 + k0 + u;
-            const float r2 = dist2(pi.x - q[u].x, pi.y - q[u].y, pi.z - q[u].z);
-            mask |= (j < c1 && j != s && r2 < h2) ? (1u << (k0 + u)) : 0u;
-        }
-    }
-    return mask;
-}
-
-// Slot ranges of the 9 stencil rows (z-major, then y), fetched together.
-// Rows outside the grid are empty.  Row (y, z) holds cells x0..x1, whose
-// slots are contiguous in cell order.
-struct Rows {
-    uint32_t a[9], b[9];
-};
-
-__device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* __restrict__ cell_start,
-                                             const GridDev& g, uint32_t cell_off, uint32_t n) {
-    Rows r;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        const int z = st.z0 + k / 3, y = st.y0 + k % 3;
-        const bool ok = z <= st.z1 && y <= st.y1;
-        const uint32_t row = cell_off + (ok ? (uint32_t)y * g.dx + (uint32_t)z * g.dxy : 0u);
-        const uint32_t a = cell_start[row + (uint32_t)st.x0];
-        const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
-        r.a[k] = ok ? a : 0u;
-        r.b[k] = ok ? b : 0u;
-    }
-    return r;
-}
-
-// One neighbour's contribution to the density sum (FP64) and to the
-// constraint gradient terms (Constraints.metal:110-119).
-__device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
-                                             double& wsum, float& gx, float& gy, float& gz,
-                                             float& gsq) {
-    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-    const float r2 = dist2(rx, ry, rz);
-    const double dxd = rx, dyd = ry, dzd = rz;
-    const double diff = sd.h2_d - (dxd * dxd + dyd * dyd + dzd * dzd);
-    wsum += diff * diff * diff;
-    // gradient terms only enter lambda's denominator: hardware 1/sqrt (~1 ulp)
-    const float inv_r = __builtin_amdgcn_rsqf(r2);
-    const float rl = r2 * inv_r;
-    if (rl > 1e-6f && rl < sd.h) {
-        const float dd = sd.h - rl;
-        const float sc = -sd.spiky * dd * dd * inv_r;
-        const float ax = sc * rx, ay = sc * ry, az = sc * rz;
-        gx += ax;
-        gy += ay;
-        gz += az;
-        gsq += ax * ax + ay * ay + az * az;
-    }
-}
-
-__device__ __forceinline__ double ipow_d(double x, uint32_t n) {
-    double result = 1.0, b = x;
-    while (n) {
-        if (n & 1u) result *= b;
-        n >>= 1;
-        if (n) b *= b;
-    }
-    return result;
-}
-
-// One neighbour's term of the position correction (Constraints.metal:171-181),
-// in FP64 from the FP32 positions and lambdas (all exact in FP64).
-__device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const float4& pj,
-                                           const SolverDev& sd, double& dx, double& dy, double& dz) {
-    const double rx = (double)pi.x - (double)pj.x;
-    const double ry = (double)pi.y - (double)pj.y;
-    const double rz = (double)pi.z - (double)pj.z;
-    const double r2 = rx * rx + ry * ry + rz * rz;
-    // 1/r from the hardware estimate (~2^-23) plus one Newton step (~1e-14);
-    // r2 = 0 gives NaN here and fails the range test below, like r <= 1e-6.
-    const double y0 = __builtin_amdgcn_rsq(r2);
-    const double inv_r = y0 * (1.5 - 0.5 * r2 * y0 * y0);
-    const double rl = r2 * inv_r;
-    double sc = 0.0;
-    if (rl > (double)1e-6f && rl < sd.h_d) {
-        const double dd = sd.h_d - rl;
-        sc = -sd.spiky_d * dd * dd * inv_r;
-    }
-    const double diff = sd.h2_d - r2;
-    const double wij = sd.poly6_d * diff * diff * diff;
-    const double scorr = -sd.scorr_k_d * ipow_d(wij * sd.inv_wq_d, sd.scorr_n);
-    const double coef = (lam_i + (double)pj.w + scorr) * sc;
-    dx += coef * rx;
-    dy += coef * ry;
-    dz += coef * rz;
-}
-
-template <bool kOrig, bool kCount>
-__device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& sc, float4* __restrict__ cur,
-                                                    const uint32_t* __restrict__ cell_start,
-                                                    const uint32_t* __restrict__ sorted_index, uint32_t n,
-                                                    float* __restrict__ lambda_orig,
-                                                    uint32_t* __restrict__ nbr_orig) {
-    const SolverDev& sd = sc.sd;
-    const float4 pi = cur[s];
-    const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
-    // sum of (h^2 - r^2)^3 in FP64; r's components are differences of
-    // nearby FP32 values (exact by Sterbenz away from the origin), so the
-    // density is accurate to FP64 rounding.
-    double wsum = 0.0;
-    float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
-    uint32_t cnt = 0;
-    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
-#pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-        const uint32_t b = rows.b[k];
-        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
-            uint32_t mask = neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sd.h2);
-            cnt += (uint32_t)__popc(mask);
-            while (mask) {
-                // two neighbours per trip: both loads are in flight together
-                const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const bool two = mask != 0u;
-                const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
-                if (two) mask &= mask - 1u;
-                const float4 q1 = cur[j1], q2 = cur[j2];
-                density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
-                if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
-            }
-        }
-    }
-    const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
-    const float denom = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq) + sd.eps;
-    const float lambda = (float)(-constraint / (double)denom);
-    reinterpret_cast<float*>(cur)[4 * (size_t)s + 3] = lambda;
-    if (kOrig || kCount) {
-        const uint32_t i = sorted_index[s];
-        if (kOrig) lambda_orig[i] = lambda;
-        if (kCount) nbr_orig[i] = cnt;
-    }
-}
 
 template <uint32_t kB, bool kOrig, bool kCount>
 __global__ __launch_bounds__(kB) void k_density_lambda(
@@ -201,52 +22,9 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     const uint32_t n = a.n;
     if (s >= n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        density_lambda_slot<kOrig, kCount>(s, sc, cur, cell_start, sorted_index, n, lambda_orig, nbr_orig);
+        density_lambda_slot<kOrig, kCount, false>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
+                                                  nbr_orig, nullptr);
     });
-}
-
-__device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
-                                                 const float4* __restrict__ cur,
-                                                 const uint32_t* __restrict__ cell_start, uint32_t n,
-                                                 float4* __restrict__ next) {
-    const SolverDev& sd = sc.sd;
-    const float4 pi = cur[s];
-    const float lam_i = pi.w;
-    const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
-    const double lam_d = lam_i;
-    double dx = 0.0, dy = 0.0, dz = 0.0;
-    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
-#pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-        const uint32_t b = rows.b[k];
-        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
-            uint32_t mask = neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sd.h2);
-            while (mask) {
-                const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const bool two = mask != 0u;
-                const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
-                if (two) mask &= mask - 1u;
-                const float4 q1 = cur[j1], q2 = cur[j2];
-                delta_term(pi, lam_d, q1, sd, dx, dy, dz);
-                if (two) delta_term(pi, lam_d, q2, sd, dx, dy, dz);
-            }
-        }
-    }
-    dx *= sd.inv_rho0_d;
-    dy *= sd.inv_rho0_d;
-    dz *= sd.inv_rho0_d;
-    const double dlen = sqrt(dx * dx + dy * dy + dz * dz);
-    if (dlen > sd.max_step_d) {
-        const double f = sd.max_step_d / dlen;
-        dx *= f;
-        dy *= f;
-        dz *= f;
-    }
-    // clamp to the box (Constraints.metal:197), one rounding to FP32
-    next[s] = make_float4((float)fmin(fmax((double)pi.x + dx, (double)sd.bminx), (double)sd.bmaxx),
-                          (float)fmin(fmax((double)pi.y + dy, (double)sd.bminy), (double)sd.bmaxy),
-                          (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
 }
 
 template <uint32_t kB>
@@ -256,65 +34,18 @@ __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ c
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t n = a.n;
     if (s >= n) return;
-    with_scene(s, a, [&](const SceneDev& sc) { apply_delta_slot(s, sc, cur, cell_start, n, next); });
+    with_scene(s, a, [&](const SceneDev& sc) { apply_delta_slot<false>(s, sc, cur, cell_start, n, next, nullptr); });
 }
 
 // Scatter back + finalize: slot s writes particle sorted_index[s].  The old
 // position (still in x) becomes prev; x and v get the step's result.
-__device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, float dt,
-                                              const float4* __restrict__ fin,
-                                              const uint32_t* __restrict__ sorted_index,
-                                              float4* __restrict__ x, float4* __restrict__ v,
-                                              float4* __restrict__ prev) {
-    const float inv_dt = 1.0f / dt;       // Engine.cpp:561
-    const float vmax = fd.vmax_h * inv_dt;  // Constraints.metal:347
-    const uint32_t i = sorted_index[s];
-    const float4 pf = fin[s];
-    const float4 old = x[i];
-    prev[i] = old;
-    const float px = fminf(fmaxf(pf.x, fd.bminx), fd.bmaxx);
-    const float py = fminf(fmaxf(pf.y, fd.bminy), fd.bmaxy);
-    const float pz = fminf(fmaxf(pf.z, fd.bminz), fd.bmaxz);
-    float vx = (px - old.x) * inv_dt * fd.damping;
-    float vy = (py - old.y) * inv_dt * fd.damping;
-    float vz = (pz - old.z) * inv_dt * fd.damping;
-    const bool xmin = px <= fd.bminx + fd.weps, xmax = px >= fd.bmaxx - fd.weps;
-    const bool ymin = py <= fd.bminy + fd.weps, ymax = py >= fd.bmaxy - fd.weps;
-    const bool zmin = pz <= fd.bminz + fd.weps, zmax = pz >= fd.bmaxz - fd.weps;
-    if (xmin) vx = fmaxf(vx, 0.f);
-    if (xmax) vx = fminf(vx, 0.f);
-    if (ymin) vy = fmaxf(vy, 0.f);
-    if (ymax) vy = fminf(vy, 0.f);
-    if (zmin) vz = fmaxf(vz, 0.f);
-    if (zmax) vz = fminf(vz, 0.f);
-    if (ymin) {
-        vx *= fd.floor_f;
-        vz *= fd.floor_f;
-    } else if (xmin || xmax || zmin || zmax) {
-        vx *= fd.wall_f;
-        vy *= fd.wall_f;
-        vz *= fd.wall_f;
-    }
-    const float vlen = sqrtf(vx * vx + vy * vy + vz * vz);
-    if (vlen > vmax) {
-        const float f = vmax / vlen;
-        vx *= f;
-        vy *= f;
-        vz *= f;
-    }
-    x[i] = make_float4(px, py, pz, 1.0f);
-    v[i] = make_float4(vx, vy, vz, 0.0f);
-}
-
 __global__ __launch_bounds__(kBlock) void k_finalize(
     const float4* __restrict__ fin, const uint32_t* __restrict__ sorted_index, StepArgs a,
-    float4* __restrict__ x, float4* __restrict__ v, float4* __restrict__ prev) {
+    float4* x, float4* __restrict__ v, float4* __restrict__ prev) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= a.n) return;
-    with_scene(s, a, [&](const SceneDev& sc) { finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, v, prev); });
+    with_scene(s, a, [&](const SceneDev& sc) { finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, x, v, prev); });
 }
-
-inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
 template <uint32_t kB>
 void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a,

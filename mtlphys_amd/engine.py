@@ -40,6 +40,15 @@ def _ptr(arr):
     return arr.ctypes.data_as(ctypes.c_void_p)
 
 
+def _dev(p):
+    """A device address: None, an int, or an object with data_ptr() (a torch tensor)."""
+    if p is None:
+        return None
+    if hasattr(p, "data_ptr"):
+        return ctypes.c_void_p(int(p.data_ptr()))
+    return ctypes.c_void_p(int(p))
+
+
 def generate_scene(scene, n, seed=0, params=None):
     """Host-side scene seeding (Engine::seedParticles, Engine.cpp:454-549).
 
@@ -224,6 +233,77 @@ class Engine:
         C.check(C.lib().mtlp_error_flags(self._h, ctypes.byref(f)), "mtlp_error_flags")
         return f.value
 
+    # ---- stepping options -------------------------------------------------
+    def step_n(self, dt, n_steps, stream=None):
+        C.check(C.lib().mtlp_step_n(self._h, _stream_handle(stream), ctypes.c_float(dt), int(n_steps)),
+                "mtlp_step_n")
+
+    def set_graph(self, on=True):
+        """Capture each step into a HIP graph (once per dt) and replay it."""
+        C.check(C.lib().mtlp_set_graph(self._h, 1 if on else 0), "mtlp_set_graph")
+
+    def autotune(self, dt, cache_path=None, stream=None):
+        """Pick the stencil workgroup size by timing; returns it."""
+        b = ctypes.c_uint32()
+        C.check(C.lib().mtlp_autotune(self._h, _stream_handle(stream), ctypes.c_float(dt),
+                                      None if cache_path is None else str(cache_path).encode(),
+                                      ctypes.byref(b)), "mtlp_autotune")
+        return b.value
+
+    def set_stencil_block(self, block):
+        C.check(C.lib().mtlp_set_stencil_block(self._h, int(block)), "mtlp_set_stencil_block")
+
+    # ---- device-resident state (pointers are device addresses, n float4) ---
+    def set_state_device(self, d_x4=None, d_v4=None, stream=None):
+        C.check(C.lib().mtlp_set_state_device(self._h, _stream_handle(stream), _dev(d_x4), _dev(d_v4)),
+                "mtlp_set_state_device")
+
+    def get_state_device(self, d_x4=None, d_v4=None, stream=None):
+        C.check(C.lib().mtlp_get_state_device(self._h, _stream_handle(stream), _dev(d_x4), _dev(d_v4)),
+                "mtlp_get_state_device")
+
+    # ---- differentiable rollout ---------------------------------------------
+    def rollout(self, dt, steps, ckpt_every, stream=None):
+        """`steps` steps keeping (x, v) every ckpt_every steps for the adjoint."""
+        C.check(C.lib().mtlp_rollout(self._h, _stream_handle(stream), ctypes.c_float(dt), int(steps),
+                                     int(ckpt_every)), "mtlp_rollout")
+
+    def adjoint_rollout(self, dt, steps, ckpt_every, d_xbar_T=None, d_vbar_T=None, d_xbar_0=None,
+                        d_vbar_0=None, d_gbar=None, stream=None):
+        """Adjoint of the last rollout: (dL/dx_T, dL/dv_T) -> dL/dx_0, dL/dv_0,
+        dL/dgravity (device pointers; None = 0 in, not written out)."""
+        C.check(C.lib().mtlp_adjoint_rollout(self._h, _stream_handle(stream), ctypes.c_float(dt), int(steps),
+                                             int(ckpt_every), _dev(d_xbar_T), _dev(d_vbar_T),
+                                             _dev(d_xbar_0), _dev(d_vbar_0), _dev(d_gbar)),
+                "mtlp_adjoint_rollout")
+
+    def density(self, d_x4, d_rho, stream=None):
+        """rho at the n float4 positions d_x4 (fresh spatial hash) into d_rho."""
+        C.check(C.lib().mtlp_density(self._h, _stream_handle(stream), _dev(d_x4), _dev(d_rho)),
+                "mtlp_density")
+
+    def density_vjp(self, d_x4, d_rhobar, d_xbar, stream=None):
+        C.check(C.lib().mtlp_density_vjp(self._h, _stream_handle(stream), _dev(d_x4), _dev(d_rhobar),
+                                         _dev(d_xbar)), "mtlp_density_vjp")
+
+    def tape_record(self, dt, steps, stream=None):
+        """`steps` taped steps from the current state, which is left as it is."""
+        C.check(C.lib().mtlp_tape_record(self._h, _stream_handle(stream), ctypes.c_float(dt), int(steps)),
+                "mtlp_tape_record")
+
+    def tape_read(self, kind, step, iteration=0):
+        """Host copy of one tape array (synchronous)."""
+        ptr = ctypes.c_void_p()
+        nbytes = ctypes.c_size_t()
+        C.check(C.lib().mtlp_tape_buffer(self._h, int(kind), int(step), int(iteration), ctypes.byref(ptr),
+                                         ctypes.byref(nbytes)), "mtlp_tape_buffer")
+        u32 = kind in (C.TAPE_SORTED_INDEX, C.TAPE_CELL_START)
+        dtype = np.uint32 if u32 else np.float32
+        out = np.empty(nbytes.value // 4, dtype)
+        C.check(C.lib().mtlp_tape_read(self._h, int(kind), int(step), int(iteration), _ptr(out), out.nbytes),
+                "mtlp_tape_read")
+        return out if u32 else out.reshape(-1, 4)
+
 
 class Batch(Engine):
     """Many independent scenes stepped together, one kernel launch per stage.
@@ -265,32 +345,3 @@ class Batch(Engine):
         C.check(C.lib().mtlp_batch_set_params(self._h, int(scene), ctypes.byref(p)),
                 "mtlp_batch_set_params")
         self.scene_params[scene] = p.copy()
-
-
-def _engine_extras():
-    def step_n(self, dt, n_steps, stream=None):
-        C.check(C.lib().mtlp_step_n(self._h, _stream_handle(stream), ctypes.c_float(dt), int(n_steps)),
-                "mtlp_step_n")
-
-    def set_graph(self, on=True):
-        """Capture each step into a HIP graph (once per dt) and replay it."""
-        C.check(C.lib().mtlp_set_graph(self._h, 1 if on else 0), "mtlp_set_graph")
-
-    def autotune(self, dt, cache_path=None, stream=None):
-        """Pick the stencil workgroup size by timing; returns it."""
-        b = ctypes.c_uint32()
-        C.check(C.lib().mtlp_autotune(self._h, _stream_handle(stream), ctypes.c_float(dt),
-                                      None if cache_path is None else str(cache_path).encode(),
-                                      ctypes.byref(b)), "mtlp_autotune")
-        return b.value
-
-    def set_stencil_block(self, block):
-        C.check(C.lib().mtlp_set_stencil_block(self._h, int(block)), "mtlp_set_stencil_block")
-
-    Engine.step_n = step_n
-    Engine.set_graph = set_graph
-    Engine.autotune = autotune
-    Engine.set_stencil_block = set_stencil_block
-
-
-_engine_extras()

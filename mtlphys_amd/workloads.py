@@ -4,6 +4,13 @@ dam_break(n)      1M-particle dam break (scene 4): h = 0.02, spacing h/2,
                   box 2.027 cbrt(n / 1e6) x same x half, 4 iterations
 cube_drop(n)      jittered cube drop (scene 3): h = 0.1, a (cbrt n)^3 lattice in
                   [0.2 L, 0.6 L]^3 of the box [0, L]^3, 3 iterations
+differentiable(n) the adjoint workload: the dam break at h = 0.04 (spacing
+                  0.02) in a box of 2 x 2.027 cbrt(n / 1e6) (1.634 at 65,536
+                  particles, grid 41 x 41 x 21) with the reference engine's
+                  constants (rest density 8000 (0.1/h)^3, epsilon 800, s_corr
+                  k 1e-4), 3 iterations, 40 steps with checkpoints every 8; loss
+                  |centroid(x_T) - target|^2 + 0.1 mean((rho/rho0 - 1)^2) with
+                  target = initial centroid + (0.2, 0, 0)
 ensemble(...)     independent cube drops with N_s = round(exp(U[ln 2048,
                   ln 65536])), rest density U[5000, 7000] (times (0.1/h)^3),
                   gravity U[-12, -8], 3 iterations, drawn from numpy's PCG64
@@ -27,6 +34,37 @@ def cube_drop(n=32_768, seed=1):
     p = C.scene_params(C.SCENE_JITTER_CUBE, n)
     x, v = generate_scene(C.SCENE_JITTER_CUBE, n, seed, p)
     return p, x, v
+
+
+ADJ_STEPS = 40
+ADJ_CKPT = 8
+ADJ_SHIFT = (0.2, 0.0, 0.0)
+
+
+def differentiable(n=65_536, seed=1):
+    """(params, x, v, target) of the adjoint workload.  Constants are the
+    reference engine's own (Engine.cpp:23-39: rest density 8000 at h = 0.1,
+    scaled by (0.1/h)^3 for the same particle mass ratio; epsilon 800;
+    s_corr k 1e-4; g = -9.81).  With the benchmark constants (s_corr k 0.1)
+    the rollout is so chaotic that dL/dv0 grows ~1e8 per 5 steps and
+    overflows FP32 long before 40 steps."""
+    p = C.scene_params(C.SCENE_JITTER_DAM, n)
+    d = C.default_params()
+    h = 0.04
+    S = 2.0 * 2.027 * float(np.cbrt(n / 1.0e6))
+    p.h = h
+    p.cell_size = h
+    p.bounds_max[0] = S
+    p.bounds_max[1] = S
+    p.bounds_max[2] = 0.5 * S
+    p.rest_density = d.rest_density * (0.1 / h) ** 3
+    p.epsilon = d.epsilon
+    p.scorr_k = d.scorr_k
+    p.gravity[1] = d.gravity[1]
+    p.iterations = 3
+    x, v = generate_scene(C.SCENE_JITTER_DAM, n, seed, p)
+    target = x[:, :3].astype(np.float64).mean(axis=0) + np.asarray(ADJ_SHIFT)
+    return p, x, v, target
 
 
 def ensemble_config(n_scenes=256, seed=3, n_min=2048, n_max=65536):
@@ -57,5 +95,5 @@ def step_bytes(n, cells, iters):
     return (100 + 44 * iters) * n + (16 + 8 * iters) * cells
 
 
-__all__ = ["DT", "dam_break", "cube_drop", "ensemble_config", "make_ensemble", "step_bytes",
+__all__ = ["DT", "ADJ_STEPS", "ADJ_CKPT", "ADJ_SHIFT", "differentiable", "dam_break", "cube_drop", "ensemble_config", "make_ensemble", "step_bytes",
            "Engine", "Batch"]

@@ -13,7 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "liboracle.so")
 SRC = [os.path.join(HERE, "pbf_oracle.c")]
-DEPS = SRC + [os.path.join(HERE, "pbf_oracle.h"), os.path.join(HERE, "pbf_solver_impl.h")]
+DEPS = SRC + [os.path.join(HERE, n) for n in ("pbf_oracle.h", "pbf_solver_impl.h", "pbf_adjoint_impl.h")]
 
 # -ffp-contract=off: predict and the cell hash are separately rounded FP32
 # operations (no FMA), which the GPU kernels reproduce bit for bit.

@@ -54,6 +54,11 @@ def _trace_struct(real):
         ("lambda_orig", P(real))]})
 
 
+class OrcTape(ctypes.Structure):
+    _fields_ = [("sorted_index", ctypes.c_void_p), ("cell_start", ctypes.c_void_p),
+                ("p", ctypes.c_void_p), ("x_in", ctypes.c_void_p)]
+
+
 TraceF64 = _trace_struct(ctypes.c_double)
 TraceF32 = _trace_struct(ctypes.c_float)
 
@@ -90,6 +95,13 @@ def lib():
                                             ctypes.POINTER(TraceF64)]),
             "orc_step_f32": (ctypes.c_int, [vp, vp, u32, PP, ctypes.c_float, vp, vp,
                                             ctypes.POINTER(TraceF32)]),
+            "orc_step_d": (ctypes.c_int, [vp, vp, u32, PP, ctypes.c_float, vp, vp,
+                                          ctypes.POINTER(OrcTape)]),
+            "orc_step_vjp_d": (ctypes.c_int, [ctypes.POINTER(OrcTape), u32, PP, ctypes.c_float,
+                                              vp, vp, vp, vp, vp, vp]),
+            "orc_step_kinks_d": (None, [ctypes.POINTER(OrcTape), u32, PP, ctypes.c_float, vp]),
+            "orc_density_at_d": (None, [vp, u32, PP, vp]),
+            "orc_density_vjp_d": (None, [vp, u32, PP, vp, vp]),
             "orc_set_threads": (None, [ctypes.c_int]),
             "orc_get_threads": (ctypes.c_int, []),
         }
@@ -236,7 +248,8 @@ def finalize(x4, v4, prev4, p, dt, prec=64):
     rdt, sfx = _real(prec)
     x = np.array(x4, rdt, copy=True, order="C")
     v = np.array(v4, rdt, copy=True, order="C")
-    getattr(lib(), "orc_finalize_" + sfx)(_p(x), _p(v), _p(f32x4(prev4)), len(x), ctypes.byref(p),
+    prev = np.ascontiguousarray(prev4, rdt)
+    getattr(lib(), "orc_finalize_" + sfx)(_p(x), _p(v), _p(prev), len(x), ctypes.byref(p),
                                           ctypes.c_float(dt))
     return x, v
 
@@ -268,3 +281,120 @@ def step(x4, v4, p, dt, prec=64, trace=False):
     if trace:
         return xo, vo, arrays
     return xo, vo
+
+
+# ---- FP64 rollout step and adjoint ------------------------------------------
+
+def f64x4(a):
+    a = np.ascontiguousarray(a, np.float64)
+    assert a.ndim == 2 and a.shape[1] == 4
+    return a
+
+
+def step_d(x4, v4, p, dt):
+    """FP64 step (predict in FP64, binning at the FP32 rounding of x*).
+    Returns (x4, v4, tape); the tape feeds step_vjp_d."""
+    x4, v4 = f64x4(x4), f64x4(v4)
+    n = len(x4)
+    g = grid(p)
+    tape = dict(sorted_index=np.empty(n, np.uint32), cell_start=np.empty(g.total_cells + 1, np.uint32),
+                p=np.empty((p.iterations + 1, n, 4), np.float64), x_in=np.empty((n, 4), np.float64))
+    xo = np.empty((n, 4), np.float64)
+    vo = np.empty((n, 4), np.float64)
+    t = OrcTape(*(_p(tape[k]) for k in ("sorted_index", "cell_start", "p", "x_in")))
+    lib().orc_step_d(_p(x4), _p(v4), n, ctypes.byref(p), ctypes.c_float(dt), _p(xo), _p(vo),
+                     ctypes.byref(t))
+    return xo, vo, tape
+
+
+def step_vjp_d(tape, p, dt, xbar_out, vbar_out, sums=None):
+    """(xbar_out, vbar_out) -> (xbar_in, vbar_in, gbar[3]) at the tape.
+    sums: None (gbar from the direct sum of pbar^0) or a float64 array of
+    the 6 running sums (sum xbar, sum vbar), updated in place."""
+    xb, vb = f64x4(xbar_out), f64x4(vbar_out)
+    n = len(xb)
+    xi = np.empty((n, 4), np.float64)
+    vi = np.empty((n, 4), np.float64)
+    gb = np.zeros(3, np.float64)
+    if sums is not None:
+        assert sums.dtype == np.float64 and sums.shape == (6,) and sums.flags.c_contiguous
+    t = OrcTape(*(_p(tape[k]) for k in ("sorted_index", "cell_start", "p", "x_in")))
+    lib().orc_step_vjp_d(ctypes.byref(t), n, ctypes.byref(p), ctypes.c_float(dt), _p(xb), _p(vb),
+                         _p(xi), _p(vi), _p(gb), _p(sums))
+    return xi, vi, gb
+
+
+def adjoint_sums(xbar, vbar):
+    """Initial running sums (sum xbar.xyz, sum vbar.xyz) for step_vjp_d."""
+    return np.concatenate([f64x4(xbar)[:, :3].sum(axis=0), f64x4(vbar)[:, :3].sum(axis=0)])
+
+
+def step_kinks_d(tape, p, dt):
+    """(n, 2 + iterations) uint64 branch signature of a taped step."""
+    n = len(tape["sorted_index"])
+    out = np.zeros((n, 2 + p.iterations), np.uint64)
+    t = OrcTape(*(_p(tape[k]) for k in ("sorted_index", "cell_start", "p", "x_in")))
+    lib().orc_step_kinks_d(ctypes.byref(t), n, ctypes.byref(p), ctypes.c_float(dt), _p(out))
+    return out
+
+
+def rollout_kinks(x0, v0, p, dt, steps):
+    """Branch signatures of every step of a rollout, stacked: (steps, n, 2 + I)."""
+    x, v = f64x4(x0), f64x4(v0)
+    sig = []
+    for _ in range(steps):
+        xn, vn, t = step_d(x, v, p, dt)
+        sig.append(step_kinks_d(t, p, dt))
+        x, v = xn, vn
+    return np.stack(sig)
+
+
+def density_at_d(x4, p):
+    x4 = f64x4(x4)
+    rho = np.empty(len(x4), np.float64)
+    lib().orc_density_at_d(_p(x4), len(x4), ctypes.byref(p), _p(rho))
+    return rho
+
+
+def density_vjp_d(x4, p, rhobar):
+    x4 = f64x4(x4)
+    out = np.empty_like(x4)
+    lib().orc_density_vjp_d(_p(x4), len(x4), ctypes.byref(p), _p(np.ascontiguousarray(rhobar, np.float64)),
+                            _p(out))
+    return out
+
+
+def rollout_loss(x0, v0, p, dt, steps, target, w_density=0.1):
+    """Loss L = |centroid(x_T) - target|^2 + w mean((rho(x_T)/rho0 - 1)^2)
+    of a `steps`-step FP64 rollout.  Returns (L, tapes, x_T)."""
+    x, v = f64x4(x0), f64x4(v0)
+    tapes = []
+    for _ in range(steps):
+        x, v, t = step_d(x, v, p, dt)
+        tapes.append(t)
+    n = len(x)
+    c = x[:, :3].mean(axis=0) - np.asarray(target, np.float64)
+    rho = density_at_d(x, p)
+    dev = rho / p.rest_density - 1.0
+    L = float(c @ c + w_density * np.mean(dev * dev))
+    return L, tapes, x
+
+
+def rollout_grad(x0, v0, p, dt, steps, target, w_density=0.1, exact_sums=True):
+    """dL/dx0, dL/dv0 (n x 4, w = 0) and dL/dgravity (3) by the adjoint
+    (gravity through the running-sum recursion unless exact_sums=False)."""
+    L, tapes, xT = rollout_loss(x0, v0, p, dt, steps, target, w_density)
+    n = len(xT)
+    c = xT[:, :3].mean(axis=0) - np.asarray(target, np.float64)
+    rho = density_at_d(xT, p)
+    dev = rho / p.rest_density - 1.0
+    xb = np.zeros((n, 4))
+    xb[:, :3] = 2.0 * c / n
+    xb += density_vjp_d(xT, p, w_density * 2.0 * dev / (n * p.rest_density))
+    vb = np.zeros((n, 4))
+    gb = np.zeros(3)
+    sums = adjoint_sums(xb, vb) if exact_sums else None
+    for t in reversed(tapes):
+        xb, vb, g1 = step_vjp_d(t, p, dt, xb, vb, sums)
+        gb += g1
+    return L, xb, vb, gb

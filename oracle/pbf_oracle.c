@@ -226,3 +226,6 @@ int orc_get_threads(void)
 #undef REAL
 #undef SFX
 #undef SQRT_OF
+
+/* ---- FP64 rollout step and adjoint (uses the f64 helpers above) -------- */
+#include "pbf_adjoint_impl.h"

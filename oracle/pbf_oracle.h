@@ -147,7 +147,7 @@ void orc_apply_delta_f32(const float* sx4, const float* lambda_sorted,
                          const orc_grid* g, const orc_params* p,
                          float* sx4_next);
 /* finalize on original-order positions x4 (after the scatter back) */
-void orc_finalize_f64(double* x4, double* v4, const float* prev4, uint32_t n,
+void orc_finalize_f64(double* x4, double* v4, const double* prev4, uint32_t n,
                       const orc_params* p, float dt);
 void orc_finalize_f32(float* x4, float* v4, const float* prev4, uint32_t n,
                       const orc_params* p, float dt);
@@ -161,6 +161,34 @@ int orc_step_f64(const float* x4, const float* v4, uint32_t n,
 int orc_step_f32(const float* x4, const float* v4, uint32_t n,
                  const orc_params* p, float dt,
                  float* x4_out, float* v4_out, orc_trace_f32* trace);
+
+/* ---- FP64 rollout step and its adjoint (pbf_adjoint_impl.h) ----------- */
+/* What the adjoint of one step needs from its forward pass. */
+typedef struct orc_tape {
+    uint32_t* sorted_index;  /* n                                            */
+    uint32_t* cell_start;    /* total_cells + 1                              */
+    double*   p;             /* (iterations + 1) * 4n, sorted iterates       */
+    double*   x_in;          /* 4n, positions at the start of the step       */
+} orc_tape;
+
+/* The step in FP64 throughout (predict included), binned at the FP32
+ * rounding of the FP64 prediction.  dt must be > 0. */
+int orc_step_d(const double* x4, const double* v4, uint32_t n, const orc_params* p, float dt,
+               double* x4_out, double* v4_out, orc_tape* tape);
+/* Vector-Jacobian product of orc_step_d at its tape: given the adjoints of
+ * (x_out, v_out), the adjoints of (x_in, v_in) and of gravity.  sums: NULL
+ * (gravity from the direct sum) or the 6 running sums of the adjoints, see
+ * pbf_adjoint_impl.h. */
+int orc_step_vjp_d(const orc_tape* tape, uint32_t n, const orc_params* p, float dt,
+                   const double* xbar_out, const double* vbar_out,
+                   double* xbar_in, double* vbar_in, double gbar[3], double* sums);
+/* Branch signature (bins, stencil cells, clamps, caps, wall flags, gates)
+ * of a taped step per particle: (2 + iterations) words each. */
+void orc_step_kinks_d(const orc_tape* tape, uint32_t n, const orc_params* p, float dt, uint64_t* out);
+/* rho at x (fresh binning of x), original order, and its VJP. */
+void orc_density_at_d(const double* x4, uint32_t n, const orc_params* p, double* rho);
+void orc_density_vjp_d(const double* x4, uint32_t n, const orc_params* p,
+                       const double* rhobar, double* xbar);
 
 /* OpenMP thread count for the per-particle stages (0 = library default). */
 void orc_set_threads(int threads);

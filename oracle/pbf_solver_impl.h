@@ -192,7 +192,7 @@ void CAT(orc_apply_delta_, SFX)(const REAL* sx4, const REAL* lambda_sorted,
 }
 
 /* finalizeStep, Constraints.metal:305-353, on original-order positions. */
-void CAT(orc_finalize_, SFX)(REAL* x4, REAL* v4, const float* prev4, uint32_t n,
+void CAT(orc_finalize_, SFX)(REAL* x4, REAL* v4, const REAL* prev4, uint32_t n,
                             const orc_params* p, float dt)
 {
     FN(consts) k;
@@ -208,7 +208,7 @@ void CAT(orc_finalize_, SFX)(REAL* x4, REAL* v4, const float* prev4, uint32_t n,
         REAL pp[3], v[3];
         for (int a = 0; a < 3; ++a) {
             pp[a] = FN(clampr)(x4[4 * i + a], k.bmin[a], k.bmax[a]);
-            v[a] = (pp[a] - (REAL)prev4[4 * i + a]) * inv_dt;
+            v[a] = (pp[a] - prev4[4 * i + a]) * inv_dt;
             v[a] *= damping;
         }
         const int xmin = pp[0] <= k.bmin[0] + weps, xmax = pp[0] >= k.bmax[0] - weps;
@@ -289,8 +289,10 @@ int CAT(orc_step_, SFX)(const float* x4, const float* v4, uint32_t n,
 
     for (uint32_t s = 0; s < n; ++s)                                 /* :777-785 */
         for (int a = 0; a < 4; ++a) x4_out[4 * (size_t)sidx[s] + a] = cur[4 * (size_t)s + a];
-    for (size_t i = 0; i < 4 * (size_t)n; ++i) v4_out[i] = (REAL)v4[i];
-    CAT(orc_finalize_, SFX)(x4_out, v4_out, x4, n, p, dt);          /* :788-796 */
+    REAL* prev = (REAL*)malloc(sizeof(REAL) * 4 * (size_t)(n ? n : 1));
+    for (size_t i = 0; i < 4 * (size_t)n; ++i) { v4_out[i] = (REAL)v4[i]; prev[i] = (REAL)x4[i]; }
+    CAT(orc_finalize_, SFX)(x4_out, v4_out, prev, n, p, dt);        /* :788-796 */
+    free(prev);
 
     free(xs); free(keys); free(cnt); free(cs); free(sidx); free(cur); free(nxt); free(lam);
     return 0;
