@@ -95,7 +95,9 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
 
 // One step from (xs[slot], vs[slot]) to (xs[slot+1], vs[slot+1]) with the
 // tape of slot: the same arithmetic as enqueue_step (Engine.cpp:685-811).
-mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t slot) {
+// With engine_outputs it also writes the engine's lambdas (last iteration)
+// and neighbour counts (iteration 0), as mtlp_step does.
+mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t slot, bool engine_outputs = false) {
     AdjointState& A = e->adj;
     const StepArgs a = e->step_args(dt, true);
     const StepTape t = tape_of(e, slot);
@@ -109,10 +111,15 @@ mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t
     MTLP_TRY(enqueue_hash(e, s, a, clk, b));
     const uint32_t I = A.tape_iters;
     for (uint32_t it = 0; it < I; ++it) {
-        HIP_TRY(launch_density_lambda_tape(t.iterate(it), t.cell_start, a, t.grad_sums(it), e->stencil_block, s));
+        float* lo = engine_outputs && it + 1 == I ? e->lambdas : nullptr;
+        uint32_t* nb = engine_outputs && it == 0 ? e->nbr : nullptr;
+        HIP_TRY(launch_density_lambda_tape(t.iterate(it), t.cell_start, t.sorted_index, a, t.grad_sums(it), lo, nb,
+                                           e->stencil_block, s));
         HIP_TRY(launch_apply_delta_tape(t.iterate(it), t.cell_start, a, t.iterate(it + 1), t.delta(it),
                                         e->stencil_block, s));
     }
+    if (engine_outputs && I == 0)
+        HIP_TRY(launch_count_neighbors(t.iterate(0), t.cell_start, t.sorted_index, a, e->nbr, s));
     HIP_TRY(launch_finalize_tape(t.iterate(I), t.sorted_index, a, t.x_in, A.xs + (slot + 1) * A.stride,
                                  A.vs + (slot + 1) * A.stride, s));
     return MTLP_OK;
@@ -155,6 +162,14 @@ mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t 
         A.n_ckpt = n_ckpt;
     }
     A.recorded = false;
+    // the last segment is stepped with the tape on, so the adjoint starts
+    // without recomputing it
+    const uint32_t last0 = (n_ckpt - 1) * ckpt_every;
+    if (e->n) {
+        HIP_TRY(hipStreamSynchronize(s));
+        MTLP_TRY(ensure_tape(e, ckpt_every));
+    }
+    A.tape_last_seg = false;
     const size_t bytes = sizeof(float4) * e->n;
     for (uint32_t t = 0; t < steps; ++t) {
         if (t % ckpt_every == 0) {
@@ -162,7 +177,28 @@ mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t 
             HIP_TRY(copy_d2d(A.ck_x + k * A.stride, e->x, bytes, s));
             HIP_TRY(copy_d2d(A.ck_v + k * A.stride, e->v, bytes, s));
         }
-        MTLP_TRY(mtlp_step(e, stream, dt));
+        if (t < last0 || e->n == 0) {
+            MTLP_TRY(mtlp_step(e, stream, dt));
+            continue;
+        }
+        if (t == last0) {
+            HIP_TRY(copy_d2d(A.xs, e->x, bytes, s));
+            HIP_TRY(copy_d2d(A.vs, e->v, bytes, s));
+        }
+        MTLP_TRY(enqueue_taped_step(e, s, dt, t - last0, t + 1 == steps));
+    }
+    if (e->n) {
+        // leave the engine's buffers as mtlp_step would
+        const uint32_t k = steps - last0;
+        const StepTape t = tape_of(e, k - 1);
+        HIP_TRY(copy_d2d(e->x, A.xs + k * A.stride, bytes, s));
+        HIP_TRY(copy_d2d(e->v, A.vs + k * A.stride, bytes, s));
+        HIP_TRY(copy_d2d(e->prev, A.xs + (k - 1) * A.stride, bytes, s));
+        HIP_TRY(copy_d2d(e->cell_start, t.cell_start, sizeof(uint32_t) * ((size_t)e->cells + 1), s));
+        HIP_TRY(copy_d2d(e->sorted_index, t.sorted_index, sizeof(uint32_t) * e->n, s));
+        HIP_TRY(copy_d2d(e->sorted[0], t.iterate(A.tape_iters), bytes, s));
+        e->final_buf = 0;
+        A.tape_last_seg = true;
     }
     A.recorded = true;
     A.dt = dt;
@@ -195,9 +231,12 @@ mtlp_status mtlp_adjoint_rollout(mtlp_engine* e, mtlp_stream stream, float dt, u
     for (uint32_t seg = n_seg; seg-- > 0;) {
         const uint32_t t0 = seg * ckpt_every;
         const uint32_t t1 = t0 + ckpt_every < steps ? t0 + ckpt_every : steps;
-        HIP_TRY(copy_d2d(A.xs, A.ck_x + (size_t)seg * A.stride, bytes, s));
-        HIP_TRY(copy_d2d(A.vs, A.ck_v + (size_t)seg * A.stride, bytes, s));
-        for (uint32_t t = t0; t < t1; ++t) MTLP_TRY(enqueue_taped_step(e, s, dt, t - t0));
+        if (!(seg + 1 == n_seg && A.tape_last_seg)) {
+            A.tape_last_seg = false;
+            HIP_TRY(copy_d2d(A.xs, A.ck_x + (size_t)seg * A.stride, bytes, s));
+            HIP_TRY(copy_d2d(A.vs, A.ck_v + (size_t)seg * A.stride, bytes, s));
+            for (uint32_t t = t0; t < t1; ++t) MTLP_TRY(enqueue_taped_step(e, s, dt, t - t0));
+        }
         for (uint32_t t = t1; t-- > t0;) {
             HIP_TRY(launch_step_vjp(tape_of(e, t - t0), b, a, A.tape_iters, e->scene_params[0].damping, A.xb[cur],
                                     A.vb[cur], A.xb[cur ^ 1], A.vb[cur ^ 1], A.gout, e->stencil_block, s));
@@ -243,6 +282,7 @@ mtlp_status mtlp_tape_record(mtlp_engine* e, mtlp_stream stream, float dt, uint3
     HIP_TRY(hipStreamSynchronize(s));
     MTLP_TRY(ensure_tape(e, steps));
     AdjointState& A = e->adj;
+    A.tape_last_seg = false;
     const size_t bytes = sizeof(float4) * e->n;
     HIP_TRY(copy_d2d(A.xs, e->x, bytes, s));
     HIP_TRY(copy_d2d(A.vs, e->v, bytes, s));

@@ -16,9 +16,10 @@
 //   B (i = c, j = s): s takes c's term on s when s is in N_c, i.e. when
 //     bin(s) lies in the stencil of c's current cell.
 // The two roles share the pair's geometry (H is even in r, G odd), so each
-// kernel evaluates them as one expression.  Own sums are FP64; a slot's own
-// result is a plain store, so the output is deterministic up to the order
-// of the rare remainder atomics.
+// kernel evaluates them as one expression.  Terms and sums are FP32 (the
+// terms' own rounding dominates; FP64 sums measured no closer to the FP64
+// adjoint); a slot's own result is a plain store, so the output is
+// deterministic up to the order of the rare remainder atomics.
 //
 // Math (r = p_i - p_j, f(r) = -k_s (h - r)^2 the spiky gradient magnitude,
 // G = f r^ (Constraints.metal:35-39), its Hessian H v = (f' - f/r)(r^.v) r^
@@ -39,15 +40,33 @@ namespace {
 
 // ---- taped forward ---------------------------------------------------------
 
-template <uint32_t kB>
+template <uint32_t kB, bool kOrig, bool kCount>
 __global__ __launch_bounds__(kB) void k_density_lambda_tape(float4* __restrict__ cur,
                                                            const uint32_t* __restrict__ cell_start,
-                                                           StepArgs a, float4* __restrict__ aux) {
+                                                           const uint32_t* __restrict__ sorted_index, StepArgs a,
+                                                           float4* __restrict__ aux, float* __restrict__ lambda_orig,
+                                                           uint32_t* __restrict__ nbr_orig) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        density_lambda_slot<false, false, true>(s, sc, cur, cell_start, nullptr, a.n, nullptr, nullptr, aux);
+        density_lambda_slot<kOrig, kCount, true>(s, sc, cur, cell_start, sorted_index, a.n, lambda_orig, nbr_orig,
+                                                 aux);
     });
+}
+
+template <uint32_t kB>
+void density_lambda_tape_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float4* aux,
+                           float* lo, uint32_t* nb, hipStream_t s) {
+    const dim3 grid(blocks_for(a.n, kB)), block(kB);
+    if (lo && nb)
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+    else if (lo)
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, false>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+    else if (nb)
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, false, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+    else
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, false, false>), grid, block, 0, s, cur, cs, sidx, a, aux, lo,
+                           nb);
 }
 
 template <uint32_t kB>
@@ -422,7 +441,7 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
         const uint32_t sn = sd.scorr_n;
         // d s_corr / d r = dsk * ratio^(n-1) * gp, gp = -6 poly6 (h^2 - r^2)^2
         const float dsk = -sk * (float)sn * inv_wq * (-6.0f * poly6);
-        double lb = 0.0, px = 0.0, py = 0.0, pz = 0.0;
+        float lb = 0.f, px = 0.f, py = 0.f, pz = 0.f;
         auto pair = [&](uint32_t c, const float4& pc, const float4& rc) {
             const PairGeo g = pair_geo(pi, pc, sd);
             const float diff = sd.h2 - g.r2;
@@ -455,9 +474,9 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
             pair(j1, q1, r1);
             if (two) pair(j2, q2, r2);
         });
-        lbar[s] = (float)lb;
+        lbar[s] = lb;
         const float4 acc = pacc[s];
-        pacc[s] = make_float4(acc.x + (float)px, acc.y + (float)py, acc.z + (float)pz, 0.f);
+        pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
     });
 }
 
@@ -479,7 +498,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
         const float4 pi = cur[s], ds4 = db[s], b0 = bins[s];
         const Cell3 bin = cell_of(b0.x, b0.y, b0.z, sc.g);
         const float gp6 = -6.0f * (float)sd.poly6_d;
-        double px = 0.0, py = 0.0, pz = 0.0;
+        float px = 0.f, py = 0.f, pz = 0.f;
         auto pair = [&](uint32_t c, const float4& pc, const float4& dc) {
             const PairGeo g = pair_geo(pi, pc, sd);
             const float diff = sd.h2 - g.r2;
@@ -505,7 +524,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             if (two) pair(j2, q2, d2);
         });
         const float4 acc = pacc[s];
-        pacc[s] = make_float4(acc.x + (float)px, acc.y + (float)py, acc.z + (float)pz, 0.f);
+        pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
     });
 }
 
@@ -596,10 +615,15 @@ __global__ __launch_bounds__(kBlock) void k_density_rho_vjp(const float4* __rest
         }                                                                                            \
     } while (0)
 
-hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                                      float4* aux, uint32_t block, hipStream_t s) {
+hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const uint32_t* sorted_index,
+                                      const StepArgs& a, float4* aux, float* lambda_orig, uint32_t* nbr_orig,
+                                      uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    MTLP_STENCIL_LAUNCH(k_density_lambda_tape, block, a.n, s, cur, cell_start, a, aux);
+    switch (block) {
+        case 128: density_lambda_tape_b<128>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
+        case 512: density_lambda_tape_b<512>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
+        default: density_lambda_tape_b<256>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
+    }
     return hipGetLastError();
 }
 

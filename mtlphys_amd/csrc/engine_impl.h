@@ -81,8 +81,10 @@ struct AdjointState {
     uint32_t n_ckpt = 0;
     float4* ck_x = nullptr;  // n_ckpt * stride
     float4* ck_v = nullptr;
-    // tape of up to tape_steps steps
+    // tape of up to tape_steps steps; tape_last_seg: it holds the last
+    // segment of the recorded rollout (taped during mtlp_rollout)
     uint32_t tape_steps = 0, tape_iters = 0, tape_cells = 0;
+    bool tape_last_seg = false;
     size_t stride = 0;       // padded particle count of every per-particle array
     float4* xs = nullptr;    // (tape_steps + 1) * stride: state at each step start
     float4* vs = nullptr;
@@ -104,6 +106,7 @@ struct AdjointState {
         dev_free(xs); dev_free(vs); dev_free(iter); dev_free(grad); dev_free(dlt);
         dev_free(sidx); dev_free(cs);
         tape_steps = tape_iters = tape_cells = 0;
+        tape_last_seg = false;
     }
     void release_ckpt() {
         dev_free(ck_x); dev_free(ck_v);

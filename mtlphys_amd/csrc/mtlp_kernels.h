@@ -161,8 +161,10 @@ struct AdjointBufs {
     uint32_t* err;  // error flag word
 };
 
-hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                                      float4* aux, uint32_t block, hipStream_t s);
+// lambda_orig / nbr_orig as in launch_density_lambda (may be NULL).
+hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const uint32_t* sorted_index,
+                                      const StepArgs& a, float4* aux, float* lambda_orig, uint32_t* nbr_orig,
+                                      uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                                    float4* next, float4* rr, uint32_t block, hipStream_t s);
 hipError_t launch_finalize_tape(const float4* fin, const uint32_t* sorted_index, const StepArgs& a,

@@ -82,6 +82,26 @@ def test_taped_forward_is_the_step(mtlp):
             assert e.error_flags() == 0
 
 
+def test_rollout_leaves_the_engine_as_steps_do(mtlp):
+    """mtlp_rollout (its last checkpoint segment runs taped) leaves every
+    engine buffer bit-identical to the same number of mtlp_step calls."""
+    p = mtlp.scene_params(4, 4096)
+    p.iterations = 3
+    x, v = mtlp.generate_scene(4, 4096, 3, p)
+    bufs = ("positions", "velocities", "prev_positions", "cell_hashes", "cell_start", "sorted_index",
+            "neighbor_counts", "lambdas", "sorted_positions")
+    for steps, every in ((5, 2), (6, 3), (1, 4)):
+        with _engine(mtlp, p, x, v) as a, _engine(mtlp, p, x, v) as b:
+            for _ in range(steps):
+                a.step(DT)
+            b.rollout(DT, steps, every)
+            for name in bufs:
+                ra = getattr(a, name + "_buffer")().read()
+                rb = getattr(b, name + "_buffer")().read()
+                np.testing.assert_array_equal(ra.view(np.uint32), rb.view(np.uint32), err_msg=(steps, every, name))
+            assert a.error_flags() == 0 and b.error_flags() == 0
+
+
 @pytest.mark.parametrize("scene", ["cube_bench", "cube_reference_constants", "dam_walls"])
 def test_step_vjp_matches_fp64_adjoint(mtlp, orc, scene):
     """One step: GPU (xbar_in, vbar_in, gbar) vs the FP64 oracle adjoint on
