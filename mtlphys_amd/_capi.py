@@ -118,6 +118,7 @@ SIGNATURES = {
     "mtlp_set_graph": (_I, [_P, _I]),
     "mtlp_autotune": (_I, [_P, _P, _F, ctypes.c_char_p, ctypes.POINTER(_U32)]),
     "mtlp_set_stencil_block": (_I, [_P, _U32]),
+    "mtlp_set_neighbor_lists": (_I, [_P, _I]),
     "mtlp_set_state_device": (_I, [_P, _P, _P, _P]),
     "mtlp_get_state_device": (_I, [_P, _P, _P, _P]),
     "mtlp_rollout": (_I, [_P, _P, _F, _U32, _U32]),

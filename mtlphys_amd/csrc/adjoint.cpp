@@ -41,6 +41,8 @@ mtlp_status ensure_tape(mtlp_engine* e, uint32_t steps) {
     HIP_TRY(dev_alloc(&A.iter, T * (I + 1) * st));
     HIP_TRY(dev_alloc(&A.grad, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.dlt, T * (I ? I : 1) * st));
+    HIP_TRY(dev_alloc(&A.lidx, T * (I ? I : 1) * kMaxList * st));
+    HIP_TRY(dev_alloc(&A.lcnt, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.sidx, T * st));
     HIP_TRY(dev_alloc(&A.cs, T * ((size_t)e->cells + 1)));
     A.tape_steps = steps;
@@ -89,6 +91,8 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
     t.iter = A.iter + (size_t)slot * (I + 1) * st;
     t.grad = A.grad + (size_t)slot * (I ? I : 1) * st;
     t.dlt = A.dlt + (size_t)slot * (I ? I : 1) * st;
+    t.lidx = A.lidx + (size_t)slot * (I ? I : 1) * kMaxList * st;
+    t.lcnt = A.lcnt + (size_t)slot * (I ? I : 1) * st;
     t.stride = st;
     return t;
 }
@@ -114,8 +118,8 @@ mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t
         float* lo = engine_outputs && it + 1 == I ? e->lambdas : nullptr;
         uint32_t* nb = engine_outputs && it == 0 ? e->nbr : nullptr;
         HIP_TRY(launch_density_lambda_tape(t.iterate(it), t.cell_start, t.sorted_index, a, t.grad_sums(it), lo, nb,
-                                           e->stencil_block, s));
-        HIP_TRY(launch_apply_delta_tape(t.iterate(it), t.cell_start, a, t.iterate(it + 1), t.delta(it),
+                                           t.list(it), e->stencil_block, s));
+        HIP_TRY(launch_apply_delta_tape(t.iterate(it), t.cell_start, a, t.iterate(it + 1), t.delta(it), t.list(it),
                                         e->stencil_block, s));
     }
     if (engine_outputs && I == 0)

@@ -45,37 +45,43 @@ __global__ __launch_bounds__(kB) void k_density_lambda_tape(float4* __restrict__
                                                            const uint32_t* __restrict__ cell_start,
                                                            const uint32_t* __restrict__ sorted_index, StepArgs a,
                                                            float4* __restrict__ aux, float* __restrict__ lambda_orig,
-                                                           uint32_t* __restrict__ nbr_orig) {
+                                                           uint32_t* __restrict__ nbr_orig, NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        density_lambda_slot<kOrig, kCount, true>(s, sc, cur, cell_start, sorted_index, a.n, lambda_orig, nbr_orig,
-                                                 aux);
+        density_lambda_slot<kOrig, kCount, true, true>(s, sc, cur, cell_start, sorted_index, a.n, lambda_orig,
+                                                       nbr_orig, aux, nl);
     });
 }
 
 template <uint32_t kB>
 void density_lambda_tape_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float4* aux,
-                           float* lo, uint32_t* nb, hipStream_t s) {
+                           float* lo, uint32_t* nb, const NbrList& nl, hipStream_t s) {
     const dim3 grid(blocks_for(a.n, kB)), block(kB);
     if (lo && nb)
-        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb,
+                           nl);
     else if (lo)
-        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, false>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, true, false>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb,
+                           nl);
     else if (nb)
-        hipLaunchKernelGGL((k_density_lambda_tape<kB, false, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda_tape<kB, false, true>), grid, block, 0, s, cur, cs, sidx, a, aux, lo, nb,
+                           nl);
     else
         hipLaunchKernelGGL((k_density_lambda_tape<kB, false, false>), grid, block, 0, s, cur, cs, sidx, a, aux, lo,
-                           nb);
+                           nb, nl);
 }
 
 template <uint32_t kB>
 __global__ __launch_bounds__(kB) void k_apply_delta_tape(const float4* __restrict__ cur,
                                                         const uint32_t* __restrict__ cell_start, StepArgs a,
-                                                        float4* __restrict__ next, float4* __restrict__ rr) {
+                                                        float4* __restrict__ next, float4* __restrict__ rr,
+                                                        NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
-    with_scene(s, a, [&](const SceneDev& sc) { apply_delta_slot<true>(s, sc, cur, cell_start, a.n, next, rr); });
+    with_scene(s, a, [&](const SceneDev& sc) {
+        apply_delta_slot<true, true>(s, sc, cur, cell_start, a.n, next, rr, nl);
+    });
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize_tape(const float4* __restrict__ fin,
@@ -429,7 +435,7 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                                                  const uint32_t* __restrict__ cell_start, StepArgs a,
                                                  const float4* __restrict__ rbar, float* __restrict__ lbar,
                                                  float* __restrict__ lrem, float4* __restrict__ pacc,
-                                                 float4* __restrict__ prem) {
+                                                 float4* __restrict__ prem, NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                 atomic_sub3(prem + c, tx, ty, tz);
             }
         };
-        for_each_neighbor(s, pi, sc, cur, cell_start, a.n, [&](uint32_t j1, uint32_t j2, bool two) {
+        for_each_neighbor_of<true>(s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
             const float4 q1 = cur[j1], q2 = cur[j2], r1 = rbar[j1], r2 = rbar[j2];
             pair(j1, q1, r1);
             if (two) pair(j2, q2, r2);
@@ -490,7 +496,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
                                                    const float4* __restrict__ bins,
                                                    const uint32_t* __restrict__ cell_start, StepArgs a,
                                                    const float4* __restrict__ db, float4* __restrict__ pacc,
-                                                   float4* __restrict__ prem) {
+                                                   float4* __restrict__ prem, NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
@@ -518,7 +524,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             pz += tz;
             if (!both) atomic_sub3(prem + c, tx, ty, tz);
         };
-        for_each_neighbor(s, pi, sc, cur, cell_start, a.n, [&](uint32_t j1, uint32_t j2, bool two) {
+        for_each_neighbor_of<true>(s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
             const float4 q1 = cur[j1], q2 = cur[j2], d1 = db[j1], d2 = db[j2];
             pair(j1, q1, d1);
             if (two) pair(j2, q2, d2);
@@ -617,20 +623,20 @@ __global__ __launch_bounds__(kBlock) void k_density_rho_vjp(const float4* __rest
 
 hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const uint32_t* sorted_index,
                                       const StepArgs& a, float4* aux, float* lambda_orig, uint32_t* nbr_orig,
-                                      uint32_t block, hipStream_t s) {
+                                      const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
-        case 128: density_lambda_tape_b<128>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
-        case 512: density_lambda_tape_b<512>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
-        default: density_lambda_tape_b<256>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, s); break;
+        case 128: density_lambda_tape_b<128>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
+        case 512: density_lambda_tape_b<512>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
+        default: density_lambda_tape_b<256>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                                   float4* next, float4* rr, uint32_t block, hipStream_t s) {
+                                   float4* next, float4* rr, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    MTLP_STENCIL_LAUNCH(k_apply_delta_tape, block, a.n, s, cur, cell_start, a, next, rr);
+    MTLP_STENCIL_LAUNCH(k_apply_delta_tape, block, a.n, s, cur, cell_start, a, next, rr, nl);
     return hipGetLastError();
 }
 
@@ -661,11 +667,11 @@ hipError_t launch_step_vjp(const StepTape& t, const AdjointBufs& b, const StepAr
         hipLaunchKernelGGL(k_delta_vjp_pre, pg, pb, 0, s, t.iterate(k), t.delta(k), a, b.pacc, b.prem, b.rbar,
                            b.gdev);
         MTLP_STENCIL_LAUNCH(k_delta_vjp, block, a.n, s, t.iterate(k), t.iterate(0), t.cell_start, a, b.rbar,
-                            b.lbar, b.lrem, b.pacc, b.prem);
+                            b.lbar, b.lrem, b.pacc, b.prem, t.list(k));
         hipLaunchKernelGGL(k_density_vjp_pre, pg, pb, 0, s, t.iterate(k), t.grad_sums(k), a, b.lbar, b.lrem,
                            b.db);
         MTLP_STENCIL_LAUNCH(k_density_vjp, block, a.n, s, t.iterate(k), t.iterate(0), t.cell_start, a, b.db,
-                            b.pacc, b.prem);
+                            b.pacc, b.prem, t.list(k));
     }
     hipLaunchKernelGGL(k_predict_vjp, pg, pb, 0, s, b.pacc, b.prem, t.sorted_index, a, xbar_in, vbar_in,
                        b.gdev, b.gpart, b.err);

@@ -209,6 +209,8 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->sorted[0], na));
     HIP_TRY(dev_alloc(&e->sorted[1], na));
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
+    HIP_TRY(dev_alloc(&e->nlist, (size_t)kMaxList * na));
+    HIP_TRY(dev_alloc(&e->ncount, na));
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));
@@ -308,9 +310,9 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
         const bool last = it + 1 == iters;
         HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a,
                                       last ? e->lambdas : nullptr, it == 0 ? e->nbr : nullptr,
-                                      e->stencil_block, s));
+                                      e->nbr_list(), e->stencil_block, s));
         clk.mark("density_lambda");
-        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1],
+        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], e->nbr_list(),
                                    e->stencil_block, s));
         clk.mark("apply_delta");
         cur ^= 1;
@@ -589,6 +591,13 @@ mtlp_status mtlp_set_graph(mtlp_engine* e, int enable) {
     DeviceGuard g(e->device);
     e->use_graph = enable != 0;
     if (!e->use_graph) e->drop_graph();
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_set_neighbor_lists(mtlp_engine* e, int enable) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (e->use_lists != (enable != 0)) e->drop_graph();
+    e->use_lists = enable != 0;
     return MTLP_OK;
 }
 

@@ -91,6 +91,8 @@ struct AdjointState {
     float4* iter = nullptr;  // tape_steps * (iters + 1) * stride
     float4* grad = nullptr;  // tape_steps * iters * stride
     float4* dlt = nullptr;   // tape_steps * iters * stride
+    uint32_t* lidx = nullptr;  // tape_steps * iters * kMaxList * stride: neighbour lists
+    uint32_t* lcnt = nullptr;  // tape_steps * iters * stride
     uint32_t* sidx = nullptr;  // tape_steps * stride
     uint32_t* cs = nullptr;    // tape_steps * (cells + 1)
     // backward work
@@ -103,7 +105,7 @@ struct AdjointState {
     float4* vb[2] = {nullptr, nullptr};
 
     void release_tape() {
-        dev_free(xs); dev_free(vs); dev_free(iter); dev_free(grad); dev_free(dlt);
+        dev_free(xs); dev_free(vs); dev_free(iter); dev_free(grad); dev_free(dlt); dev_free(lidx); dev_free(lcnt);
         dev_free(sidx); dev_free(cs);
         tape_steps = tape_iters = tape_cells = 0;
         tape_last_seg = false;
@@ -160,6 +162,10 @@ struct mtlp_engine {
     float4* sorted[2] = {nullptr, nullptr};
     uint32_t* big_list = nullptr;
     int final_buf = 0;
+    // neighbour lists of the current iterate (kMaxList * n_alloc) and counts
+    uint32_t* nlist = nullptr;
+    uint32_t* ncount = nullptr;
+    bool use_lists = true;
     // cell buffers
     uint32_t* counts = nullptr;
     uint32_t* cell_start = nullptr;
@@ -193,7 +199,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list); dev_free(nlist); dev_free(ncount);
         n_alloc = 0;
     }
     void free_cells() {
@@ -219,6 +225,10 @@ struct mtlp_engine {
         b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         return b;
+    }
+
+    NbrList nbr_list() const {
+        return use_lists ? NbrList{nlist, ncount, n_alloc} : NbrList{nullptr, nullptr, 0u};
     }
 
     StepArgs step_args(float dt, bool predict) const {

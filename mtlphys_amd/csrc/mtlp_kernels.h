@@ -114,16 +114,32 @@ hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool z
                                  uint32_t* err, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
+// Neighbour list of one solver iterate: the density pass records each
+// slot's neighbours in its traversal order, and the passes that follow on
+// the same iterate (delta, and the adjoint's) walk the list instead of
+// filtering the 27-cell stencil again; the order is kept, so their sums
+// are bit-identical either way.  idx[k * stride + s] is the k-th neighbour
+// of slot s for k < kMaxList; count[s] may exceed kMaxList, and then
+// readers walk the stencil for that slot.
+constexpr uint32_t kMaxList = 64;
+struct NbrList {
+    uint32_t* idx;    // kMaxList * stride
+    uint32_t* count;  // stride
+    uint32_t stride;
+};
+
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
 constexpr uint32_t kStencilBlocks[] = {128, 256, 512};
 constexpr uint32_t kNumStencilBlocks = 3;
 
+// With nl.idx non-NULL the density pass writes the list and the delta pass
+// reads it.
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  const uint32_t* sorted_index, const StepArgs& a,
-                                 float* lambda_orig, uint32_t* nbr_orig, uint32_t block,
-                                 hipStream_t s);
+                                 float* lambda_orig, uint32_t* nbr_orig, const NbrList& nl,
+                                 uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                              float4* next, uint32_t block, hipStream_t s);
+                              float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
                            const StepArgs& a, float4* x, float4* v, float4* prev, hipStream_t s);
 
@@ -139,10 +155,15 @@ struct StepTape {
     float4* iter;            // I + 1 arrays of `stride` float4
     float4* grad;            // I arrays
     float4* dlt;             // I arrays
+    uint32_t* lidx;          // I neighbour lists (kMaxList * stride each)
+    uint32_t* lcnt;          // I count arrays
     size_t stride;
     float4* iterate(uint32_t k) const { return iter + k * stride; }
     float4* grad_sums(uint32_t k) const { return grad + k * stride; }
     float4* delta(uint32_t k) const { return dlt + k * stride; }
+    NbrList list(uint32_t k) const {
+        return NbrList{lidx + (size_t)k * kMaxList * stride, lcnt + k * stride, (uint32_t)stride};
+    }
 };
 
 // Work buffers of the backward pass (sorted order unless noted).  prem and
@@ -164,9 +185,9 @@ struct AdjointBufs {
 // lambda_orig / nbr_orig as in launch_density_lambda (may be NULL).
 hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, const uint32_t* sorted_index,
                                       const StepArgs& a, float4* aux, float* lambda_orig, uint32_t* nbr_orig,
-                                      uint32_t block, hipStream_t s);
+                                      const NbrList& nl, uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                                   float4* next, float4* rr, uint32_t block, hipStream_t s);
+                                   float4* next, float4* rr, const NbrList& nl, uint32_t block, hipStream_t s);
 hipError_t launch_finalize_tape(const float4* fin, const uint32_t* sorted_index, const StepArgs& a,
                                 const float4* x_in, float4* x_out, float4* v_out, hipStream_t s);
 // Starts the gravity recursion from the adjoint inputs (xbar_T, vbar_T).

@@ -106,6 +106,38 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, 
     }
 }
 
+// Visits the neighbours recorded in nl for slot s (cnt <= kMaxList of
+// them), in the recorded order, two per trip like for_each_neighbor.
+template <class Body>
+__device__ __forceinline__ void for_each_listed(uint32_t s, uint32_t cnt, const NbrList& nl, Body&& body) {
+    const uint32_t* p = nl.idx + s;
+    const size_t st = nl.stride;
+#pragma unroll 1
+    for (uint32_t k = 0; k < cnt; k += 2) {
+        const bool two = k + 1 < cnt;
+        const uint32_t j1 = p[k * st];
+        const uint32_t j2 = p[(two ? k + 1 : k) * st];
+        body(j1, j2, two);
+    }
+}
+
+// The neighbours of slot s: from the list when it holds them all (kList
+// and count <= kMaxList), else from the stencil.  Same order either way.
+template <bool kList, class Body>
+__device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& pi, const SceneDev& sc,
+                                                     const float4* __restrict__ cur,
+                                                     const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                     const NbrList& nl, Body&& body) {
+    if (kList) {
+        const uint32_t cnt = nl.count[s];
+        if (cnt <= kMaxList) {
+            for_each_listed(s, cnt, nl, body);
+            return;
+        }
+    }
+    for_each_neighbor(s, pi, sc, cur, cell_start, n, body);
+}
+
 // One neighbour's contribution to the density sum (FP64) and to the
 // constraint gradient terms (Constraints.metal:110-119).
 __device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
@@ -170,14 +202,14 @@ __device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const
 // densityLambdaSorted for slot s: lambda_s into cur[s].w.  With kTape the
 // constraint-gradient sums the adjoint needs go to aux[s] = (sum_j grad W,
 // D) with D = (|sum_j grad W|^2 + sum_j |grad W|^2) / rho0^2, so that
-// lambda = -C / (D + eps).
-template <bool kOrig, bool kCount, bool kTape>
+// lambda = -C / (D + eps).  With kList the neighbours go to nl.
+template <bool kOrig, bool kCount, bool kTape, bool kList>
 __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& sc, float4* __restrict__ cur,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ sorted_index, uint32_t n,
                                                     float* __restrict__ lambda_orig,
                                                     uint32_t* __restrict__ nbr_orig,
-                                                    float4* __restrict__ aux) {
+                                                    float4* __restrict__ aux, const NbrList& nl) {
     const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     // sum of (h^2 - r^2)^3 in FP64; r's components are differences of
@@ -186,12 +218,19 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     double wsum = 0.0;
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
+    uint32_t* lp = kList ? nl.idx + s : nullptr;
+    const size_t lst = nl.stride;
     for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
         const float4 q1 = cur[j1], q2 = cur[j2];
+        if (kList) {
+            if (cnt < kMaxList) lp[cnt * lst] = j1;
+            if (two && cnt + 1 < kMaxList) lp[(cnt + 1) * lst] = j2;
+        }
         cnt += two ? 2u : 1u;
         density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
         if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
     });
+    if (kList) nl.count[s] = cnt;
     const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
     const float dterm = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq);
     const float denom = dterm + sd.eps;
@@ -207,17 +246,19 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
 
 // applyDeltaSorted for slot s: next[s] = clamp(p_s + cap(R_s)) with
 // R_s = sum_j (lambda_s + lambda_j + s_corr) grad W / rho0.  With kTape R_s
-// (before the 0.3 h cap) goes to rr[s] for the adjoint.
-template <bool kTape>
+// (before the 0.3 h cap) goes to rr[s] for the adjoint.  With kList the
+// neighbours come from the density pass's list.
+template <bool kTape, bool kList>
 __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
                                                  const float4* __restrict__ cur,
                                                  const uint32_t* __restrict__ cell_start, uint32_t n,
-                                                 float4* __restrict__ next, float4* __restrict__ rr) {
+                                                 float4* __restrict__ next, float4* __restrict__ rr,
+                                                 const NbrList& nl) {
     const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     const double lam_d = pi.w;
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
+    for_each_neighbor_of<kList>(s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
         const float4 q1 = cur[j1], q2 = cur[j2];
         delta_term(pi, lam_d, q1, sd, dx, dy, dz);
         if (two) delta_term(pi, lam_d, q2, sd, dx, dy, dz);

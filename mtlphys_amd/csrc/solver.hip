@@ -13,28 +13,30 @@ namespace mtlp {
 
 namespace {
 
-template <uint32_t kB, bool kOrig, bool kCount>
+template <uint32_t kB, bool kOrig, bool kCount, bool kList>
 __global__ __launch_bounds__(kB) void k_density_lambda(
     float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
-    uint32_t* __restrict__ nbr_orig) {
+    uint32_t* __restrict__ nbr_orig, NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t n = a.n;
     if (s >= n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        density_lambda_slot<kOrig, kCount, false>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
-                                                  nbr_orig, nullptr);
+        density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
+                                                         nbr_orig, nullptr, nl);
     });
 }
 
-template <uint32_t kB>
+template <uint32_t kB, bool kList>
 __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ cur,
                                                    const uint32_t* __restrict__ cell_start,
-                                                   StepArgs a, float4* __restrict__ next) {
+                                                   StepArgs a, float4* __restrict__ next, NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t n = a.n;
     if (s >= n) return;
-    with_scene(s, a, [&](const SceneDev& sc) { apply_delta_slot<false>(s, sc, cur, cell_start, n, next, nullptr); });
+    with_scene(s, a, [&](const SceneDev& sc) {
+        apply_delta_slot<false, kList>(s, sc, cur, cell_start, n, next, nullptr, nl);
+    });
 }
 
 // Scatter back + finalize: slot s writes particle sorted_index[s].  The old
@@ -47,51 +49,60 @@ __global__ __launch_bounds__(kBlock) void k_finalize(
     with_scene(s, a, [&](const SceneDev& sc) { finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, x, v, prev); });
 }
 
-template <uint32_t kB>
+template <uint32_t kB, bool kList>
 void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a,
-                      float* lo, uint32_t* nb, hipStream_t s) {
+                      float* lo, uint32_t* nb, const NbrList& nl, hipStream_t s) {
     const dim3 grid(blocks_for(a.n, kB)), block(kB);
     if (lo && nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, true>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, true, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb, nl);
     else if (lo)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, false>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, false, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
+                           nl);
     else if (nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, false, true>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, true, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
+                           nl);
     else
-        hipLaunchKernelGGL((k_density_lambda<kB, false, false>), grid, block, 0, s, cur, cs, sidx, a, lo, nb);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, false, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
+                           nl);
+}
+
+template <uint32_t kB>
+void density_lambda_bl(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float* lo,
+                       uint32_t* nb, const NbrList& nl, hipStream_t s) {
+    if (nl.idx) density_lambda_b<kB, true>(cur, cs, sidx, a, lo, nb, nl, s);
+    else density_lambda_b<kB, false>(cur, cs, sidx, a, lo, nb, nl, s);
+}
+
+template <uint32_t kB>
+void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
+                   hipStream_t s) {
+    const dim3 grid(blocks_for(a.n, kB)), block(kB);
+    if (nl.idx) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
+    else hipLaunchKernelGGL((k_apply_delta<kB, false>), grid, block, 0, s, cur, cs, a, next, nl);
 }
 
 }  // namespace
 
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  const uint32_t* sorted_index, const StepArgs& a,
-                                 float* lambda_orig, uint32_t* nbr_orig, uint32_t block,
+                                 float* lambda_orig, uint32_t* nbr_orig, const NbrList& nl, uint32_t block,
                                  hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
-        case 128: density_lambda_b<128>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
-        case 512: density_lambda_b<512>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
-        default: density_lambda_b<256>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, s); break;
+        case 128: density_lambda_bl<128>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
+        case 512: density_lambda_bl<512>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
+        default: density_lambda_bl<256>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
-                              float4* next, uint32_t block, hipStream_t s) {
+                              float4* next, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
-        case 128:
-            hipLaunchKernelGGL(k_apply_delta<128>, dim3(blocks_for(a.n, 128)), dim3(128), 0, s, cur,
-                               cell_start, a, next);
-            break;
-        case 512:
-            hipLaunchKernelGGL(k_apply_delta<512>, dim3(blocks_for(a.n, 512)), dim3(512), 0, s, cur,
-                               cell_start, a, next);
-            break;
-        default:
-            hipLaunchKernelGGL(k_apply_delta<256>, dim3(blocks_for(a.n, 256)), dim3(256), 0, s, cur,
-                               cell_start, a, next);
-            break;
+        case 128: apply_delta_b<128>(cur, cell_start, a, next, nl, s); break;
+        case 512: apply_delta_b<512>(cur, cell_start, a, next, nl, s); break;
+        default: apply_delta_b<256>(cur, cell_start, a, next, nl, s); break;
     }
     return hipGetLastError();
 }

@@ -253,6 +253,10 @@ class Engine:
     def set_stencil_block(self, block):
         C.check(C.lib().mtlp_set_stencil_block(self._h, int(block)), "mtlp_set_stencil_block")
 
+    def set_neighbor_lists(self, on=True):
+        """Density pass records neighbour lists for the delta pass (default on)."""
+        C.check(C.lib().mtlp_set_neighbor_lists(self._h, 1 if on else 0), "mtlp_set_neighbor_lists")
+
     # ---- device-resident state (pointers are device addresses, n float4) ---
     def set_state_device(self, d_x4=None, d_v4=None, stream=None):
         C.check(C.lib().mtlp_set_state_device(self._h, _stream_handle(stream), _dev(d_x4), _dev(d_v4)),

@@ -141,6 +141,43 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
             e.set_stencil_block(100)
 
 
+def test_neighbor_lists_bitwise(mtlp):
+    """Delta passes walking the density pass's neighbour lists give the same
+    bits as walking the stencil, including slots with more than 64
+    neighbours (a dense blob: they fall back to the stencil), on a single
+    scene and on a batch."""
+    p = mtlp.scene_params(3, 20000)
+    x, v = mtlp.generate_scene(3, 20000, 4, p)
+    rng = np.random.default_rng(6)
+    blob = x[:400, :3].mean(axis=0) + rng.uniform(-0.3, 0.3, (400, 3)) * p.h
+    x[:400, :3] = blob.astype(np.float32)
+    outs = {}
+    for lists in (False, True):
+        with mtlp.Engine(0, p) as e:
+            e.reset(20000, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+            e.set_neighbor_lists(lists)
+            e.step(DT)
+            nbr = e.neighbor_counts_buffer().read()
+            e.step(DT)
+            outs[lists] = (e.positions_buffer().read(), e.velocities_buffer().read(), e.lambdas_buffer().read())
+            assert e.error_flags() == 0
+    assert nbr.max() > 64 and (nbr > 64).sum() >= 100  # the fallback path ran
+    for a, b in zip(outs[False], outs[True]):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    counts, params = _mixed_scenes(mtlp)
+    got = {}
+    for lists in (False, True):
+        b = mtlp.Batch(counts, params)
+        b.seed(3, 11)
+        b.set_neighbor_lists(lists)
+        for _ in range(2):
+            b.step(DT)
+        got[lists] = b.positions_buffer().read()
+        b.close()
+    np.testing.assert_array_equal(got[False].view(np.uint32), got[True].view(np.uint32))
+
+
 def test_ensemble_256_scenes(mtlp):
     from mtlphys_amd import workloads as W
     b = W.make_ensemble()
