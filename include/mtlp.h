@@ -220,9 +220,9 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
 /* Workgroup size of the stencil kernels (128, 256 or 512). */
 mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
 /* Neighbour lists (on by default): each iteration's density pass records
- * up to 64 neighbours per particle (256 B of device memory per particle)
- * and the delta pass walks them instead of the 27-cell stencil.  Results
- * are bit-identical with lists on or off. */
+ * up to 64 neighbours per particle (260 B of device memory per particle)
+ * and the delta pass walks them instead of testing the 27-cell stencil's
+ * candidates again.  Results are bit-identical with lists on or off. */
 mtlp_status mtlp_set_neighbor_lists(mtlp_engine* e, int enable);
 
 /* ---- differentiable rollout (single-scene engines) --------------------

@@ -41,7 +41,7 @@ mtlp_status ensure_tape(mtlp_engine* e, uint32_t steps) {
     HIP_TRY(dev_alloc(&A.iter, T * (I + 1) * st));
     HIP_TRY(dev_alloc(&A.grad, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.dlt, T * (I ? I : 1) * st));
-    HIP_TRY(dev_alloc(&A.lidx, T * (I ? I : 1) * kMaxList * st));
+    HIP_TRY(dev_alloc(&A.lblk, T * (I ? I : 1) * kListBlocks * st));
     HIP_TRY(dev_alloc(&A.lcnt, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.sidx, T * st));
     HIP_TRY(dev_alloc(&A.cs, T * ((size_t)e->cells + 1)));
@@ -91,7 +91,7 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
     t.iter = A.iter + (size_t)slot * (I + 1) * st;
     t.grad = A.grad + (size_t)slot * (I ? I : 1) * st;
     t.dlt = A.dlt + (size_t)slot * (I ? I : 1) * st;
-    t.lidx = A.lidx + (size_t)slot * (I ? I : 1) * kMaxList * st;
+    t.lblk = A.lblk + (size_t)slot * (I ? I : 1) * kListBlocks * st;
     t.lcnt = A.lcnt + (size_t)slot * (I ? I : 1) * st;
     t.stride = st;
     return t;

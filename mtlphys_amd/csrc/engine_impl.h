@@ -91,7 +91,7 @@ struct AdjointState {
     float4* iter = nullptr;  // tape_steps * (iters + 1) * stride
     float4* grad = nullptr;  // tape_steps * iters * stride
     float4* dlt = nullptr;   // tape_steps * iters * stride
-    uint32_t* lidx = nullptr;  // tape_steps * iters * kMaxList * stride: neighbour lists
+    uint4* lblk = nullptr;     // tape_steps * iters * kListBlocks * stride: neighbour lists
     uint32_t* lcnt = nullptr;  // tape_steps * iters * stride
     uint32_t* sidx = nullptr;  // tape_steps * stride
     uint32_t* cs = nullptr;    // tape_steps * (cells + 1)
@@ -105,7 +105,7 @@ struct AdjointState {
     float4* vb[2] = {nullptr, nullptr};
 
     void release_tape() {
-        dev_free(xs); dev_free(vs); dev_free(iter); dev_free(grad); dev_free(dlt); dev_free(lidx); dev_free(lcnt);
+        dev_free(xs); dev_free(vs); dev_free(iter); dev_free(grad); dev_free(dlt); dev_free(lblk); dev_free(lcnt);
         dev_free(sidx); dev_free(cs);
         tape_steps = tape_iters = tape_cells = 0;
         tape_last_seg = false;
@@ -162,8 +162,8 @@ struct mtlp_engine {
     float4* sorted[2] = {nullptr, nullptr};
     uint32_t* big_list = nullptr;
     int final_buf = 0;
-    // neighbour lists of the current iterate (kMaxList * n_alloc) and counts
-    uint32_t* nlist = nullptr;
+    // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
+    uint4* nlist = nullptr;
     uint32_t* ncount = nullptr;
     bool use_lists = true;
     // cell buffers

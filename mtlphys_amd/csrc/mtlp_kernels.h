@@ -115,15 +115,17 @@ hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool z
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
 // Neighbour list of one solver iterate: the density pass records each
-// slot's neighbours in its traversal order, and the passes that follow on
+// slot's neighbours in its stencil-walk order, and the passes that follow on
 // the same iterate (delta, and the adjoint's) walk the list instead of
-// filtering the 27-cell stencil again; the order is kept, so their sums
-// are bit-identical either way.  idx[k * stride + s] is the k-th neighbour
-// of slot s for k < kMaxList; count[s] may exceed kMaxList, and then
-// readers walk the stencil for that slot.
+// testing the 27-cell stencil's candidates again; the order is kept, so
+// their sums are bit-identical either way.  Entries go in blocks of four
+// (one 16-byte store per lane, blk[b * stride + s] holds entries 4b..4b+3
+// of slot s) so a wave's stores fill whole cache lines; count[s] may exceed
+// kMaxList, and then readers walk the stencil for that slot.
 constexpr uint32_t kMaxList = 64;
+constexpr uint32_t kListBlocks = kMaxList / 4;
 struct NbrList {
-    uint32_t* idx;    // kMaxList * stride
+    uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
     uint32_t stride;
 };
@@ -132,7 +134,7 @@ struct NbrList {
 constexpr uint32_t kStencilBlocks[] = {128, 256, 512};
 constexpr uint32_t kNumStencilBlocks = 3;
 
-// With nl.idx non-NULL the density pass writes the list and the delta pass
+// With nl.blk non-NULL the density pass writes the list and the delta pass
 // reads it.
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  const uint32_t* sorted_index, const StepArgs& a,
@@ -155,14 +157,14 @@ struct StepTape {
     float4* iter;            // I + 1 arrays of `stride` float4
     float4* grad;            // I arrays
     float4* dlt;             // I arrays
-    uint32_t* lidx;          // I neighbour lists (kMaxList * stride each)
+    uint4* lblk;             // I neighbour lists (kListBlocks * stride each)
     uint32_t* lcnt;          // I count arrays
     size_t stride;
     float4* iterate(uint32_t k) const { return iter + k * stride; }
     float4* grad_sums(uint32_t k) const { return grad + k * stride; }
     float4* delta(uint32_t k) const { return dlt + k * stride; }
     NbrList list(uint32_t k) const {
-        return NbrList{lidx + (size_t)k * kMaxList * stride, lcnt + k * stride, (uint32_t)stride};
+        return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride};
     }
 };
 

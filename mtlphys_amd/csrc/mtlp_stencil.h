@@ -81,7 +81,7 @@ __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* 
 
 // Visits every neighbour j of slot s (|p_s - p_j|^2 < h^2 in FP32, j != s,
 // j in the 27-cell stencil of p_s's cell), two per trip so both loads are in
-// flight together: body(j) is called once per neighbour, in slot order.
+// flight together: body(j1, j2, two) per trip, in slot order.
 template <class Body>
 __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
@@ -106,23 +106,35 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, 
     }
 }
 
-// Visits the neighbours recorded in nl for slot s (cnt <= kMaxList of
-// them), in the recorded order, two per trip like for_each_neighbor.
-template <class Body>
-__device__ __forceinline__ void for_each_listed(uint32_t s, uint32_t cnt, const NbrList& nl, Body&& body) {
-    const uint32_t* p = nl.idx + s;
-    const size_t st = nl.stride;
-#pragma unroll 1
-    for (uint32_t k = 0; k < cnt; k += 2) {
-        const bool two = k + 1 < cnt;
-        const uint32_t j1 = p[k * st];
-        const uint32_t j2 = p[(two ? k + 1 : k) * st];
-        body(j1, j2, two);
-    }
-}
+// Records a slot's neighbours in blocks of four (see NbrList): push() the
+// walk's neighbours in order, then finish().
+struct ListWriter {
+    uint4* out;       // nl.blk + s
+    uint32_t stride;
+    uint32_t cnt = 0;
+    uint32_t e0 = 0, e1 = 0, e2 = 0;
 
-// The neighbours of slot s: from the list when it holds them all (kList
-// and count <= kMaxList), else from the stencil.  Same order either way.
+    __device__ __forceinline__ void push(uint32_t j) {
+        const uint32_t f = cnt & 3u;
+        if (f == 3u) {
+            const uint32_t b = cnt >> 2;
+            if (b < kListBlocks) out[(size_t)b * stride] = make_uint4(e0, e1, e2, j);
+        } else {
+            e0 = f == 0u ? j : e0;
+            e1 = f == 1u ? j : e1;
+            e2 = f == 2u ? j : e2;
+        }
+        ++cnt;
+    }
+    __device__ __forceinline__ void finish(uint32_t* count, uint32_t s) {
+        const uint32_t b = cnt >> 2;
+        if ((cnt & 3u) && b < kListBlocks) out[(size_t)b * stride] = make_uint4(e0, e1, e2, 0u);
+        count[s] = cnt;
+    }
+};
+
+// The neighbours of slot s in walk order: from the list when it holds them
+// all (kList and count <= kMaxList), else from the stencil.
 template <bool kList, class Body>
 __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& pi, const SceneDev& sc,
                                                      const float4* __restrict__ cur,
@@ -131,7 +143,14 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
     if (kList) {
         const uint32_t cnt = nl.count[s];
         if (cnt <= kMaxList) {
-            for_each_listed(s, cnt, nl, body);
+            const uint4* p = nl.blk + s;
+            const size_t st = nl.stride;
+#pragma unroll 1
+            for (uint32_t k = 0; k < cnt; k += 4) {
+                const uint4 e = p[(size_t)(k >> 2) * st];
+                body(e.x, e.y, k + 1 < cnt);
+                if (k + 2 < cnt) body(e.z, e.w, k + 3 < cnt);
+            }
             return;
         }
     }
@@ -218,19 +237,18 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     double wsum = 0.0;
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
-    uint32_t* lp = kList ? nl.idx + s : nullptr;
-    const size_t lst = nl.stride;
+    ListWriter lw{kList ? nl.blk + s : nullptr, nl.stride};
     for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
         const float4 q1 = cur[j1], q2 = cur[j2];
         if (kList) {
-            if (cnt < kMaxList) lp[cnt * lst] = j1;
-            if (two && cnt + 1 < kMaxList) lp[(cnt + 1) * lst] = j2;
+            lw.push(j1);
+            if (two) lw.push(j2);
         }
         cnt += two ? 2u : 1u;
         density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
         if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
     });
-    if (kList) nl.count[s] = cnt;
+    if (kList) lw.finish(nl.count, s);
     const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
     const float dterm = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq);
     const float denom = dterm + sd.eps;

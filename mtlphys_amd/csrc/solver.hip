@@ -69,7 +69,7 @@ void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, con
 template <uint32_t kB>
 void density_lambda_bl(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float* lo,
                        uint32_t* nb, const NbrList& nl, hipStream_t s) {
-    if (nl.idx) density_lambda_b<kB, true>(cur, cs, sidx, a, lo, nb, nl, s);
+    if (nl.blk) density_lambda_b<kB, true>(cur, cs, sidx, a, lo, nb, nl, s);
     else density_lambda_b<kB, false>(cur, cs, sidx, a, lo, nb, nl, s);
 }
 
@@ -77,7 +77,7 @@ template <uint32_t kB>
 void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
                    hipStream_t s) {
     const dim3 grid(blocks_for(a.n, kB)), block(kB);
-    if (nl.idx) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
+    if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
     else hipLaunchKernelGGL((k_apply_delta<kB, false>), grid, block, 0, s, cur, cs, a, next, nl);
 }
 

@@ -162,7 +162,7 @@ def test_neighbor_lists_bitwise(mtlp):
             e.step(DT)
             outs[lists] = (e.positions_buffer().read(), e.velocities_buffer().read(), e.lambdas_buffer().read())
             assert e.error_flags() == 0
-    assert nbr.max() > 64 and (nbr > 64).sum() >= 100  # the fallback path ran
+    assert (nbr > 64).sum() >= 100  # the fallback path ran
     for a, b in zip(outs[False], outs[True]):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     counts, params = _mixed_scenes(mtlp)
