@@ -38,7 +38,7 @@ mtlp_status ensure_tape(mtlp_engine* e, uint32_t steps) {
     const size_t T = steps;
     HIP_TRY(dev_alloc(&A.xs, (T + 1) * st));
     HIP_TRY(dev_alloc(&A.vs, (T + 1) * st));
-    HIP_TRY(dev_alloc(&A.iter, T * (I + 1) * st));
+    HIP_TRY(dev_alloc(&A.iter, T * (I + 1) * st + kChunkSlack));  // the stencil filter reads past the end
     HIP_TRY(dev_alloc(&A.grad, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.dlt, T * (I ? I : 1) * st));
     HIP_TRY(dev_alloc(&A.lblk, T * (I ? I : 1) * kListBlocks * st));

@@ -206,8 +206,8 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->lambdas, na));
     HIP_TRY(dev_alloc(&e->tmp_index, na));
     HIP_TRY(dev_alloc(&e->sorted_index, na));
-    HIP_TRY(dev_alloc(&e->sorted[0], na));
-    HIP_TRY(dev_alloc(&e->sorted[1], na));
+    HIP_TRY(dev_alloc(&e->sorted[0], na + kChunkSlack));  // the stencil filter reads past the end
+    HIP_TRY(dev_alloc(&e->sorted[1], na + kChunkSlack));
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
     HIP_TRY(dev_alloc(&e->nlist, (size_t)kListBlocks * na));
     HIP_TRY(dev_alloc(&e->ncount, na));
@@ -218,8 +218,8 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(hipMemset(e->nbr, 0, sizeof(uint32_t) * na));
     HIP_TRY(hipMemset(e->lambdas, 0, sizeof(float) * na));
     HIP_TRY(hipMemset(e->sorted_index, 0, sizeof(uint32_t) * na));
-    HIP_TRY(hipMemset(e->sorted[0], 0, sizeof(float4) * na));
-    HIP_TRY(hipMemset(e->sorted[1], 0, sizeof(float4) * na));
+    HIP_TRY(hipMemset(e->sorted[0], 0, sizeof(float4) * (na + kChunkSlack)));
+    HIP_TRY(hipMemset(e->sorted[1], 0, sizeof(float4) * (na + kChunkSlack)));
     e->n_alloc = na;
     e->n = n;
     return MTLP_OK;

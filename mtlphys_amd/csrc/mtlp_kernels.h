@@ -26,6 +26,7 @@ constexpr uint32_t kSmallCell = 256;      // cells up to this size are ranked in
 constexpr uint32_t kLdsSortMax = 16384;   // cells up to this size: LDS bitonic sort
 constexpr uint32_t kBigBlock = 1024;      // workgroup of the big-cell kernel
 constexpr uint32_t kBigGrid = 128;        // workgroups of the big-cell kernel
+constexpr uint32_t kChunkSlack = 32;      // readable slots past the end of every sorted position buffer
 
 // Error flag bits (device word, read by mtlp_error_flags).
 constexpr uint32_t kErrScanLookback = 1u;  // a scan tile gave up waiting

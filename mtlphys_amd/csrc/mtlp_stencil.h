@@ -35,25 +35,29 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
 // set when slot c0 + k (k < c1 - c0 <= 32) is a neighbour of s, i.e.
 // |p_s - p_j|^2 < h^2 and j != s.  The expensive terms then run only on set
 // bits, so lanes of a wave stay busy on real neighbours.  Loads go out eight
-// at a time (addresses clamped into [c0, c1)) before any is consumed.
+// at a time from one base address before any is consumed; they may read up
+// to kChunkSlack slots past the last particle (position buffers carry that
+// slack), and the bits of those and of s itself are masked off at the end.
 __device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur, uint32_t c0,
                                                   uint32_t c1, uint32_t s, float4 pi, float h2) {
+    const float4* q = cur + c0;
+    const uint32_t len = c1 - c0;
     uint32_t mask = 0;
-    const uint32_t last = c1 - 1;
 #pragma unroll
     for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
-        if (c0 + k0 >= c1) break;
-        float4 q[8];
+        if (k0 >= len) break;
+        float4 v[8];
 #pragma unroll
-        for (uint32_t u = 0; u < 8; ++u) q[u] = cur[min(c0 + k0 + u, last)];
+        for (uint32_t u = 0; u < 8; ++u) v[u] = q[k0 + u];
 #pragma unroll
         for (uint32_t u = 0; u < 8; ++u) {
-            const uint32_t j = c0 + k0 + u;
-            const float r2 = dist2(pi.x - q[u].x, pi.y - q[u].y, pi.z - q[u].z);
-            mask |= (j < c1 && j != s && r2 < h2) ? (1u << (k0 + u)) : 0u;
+            const float r2 = dist2(pi.x - v[u].x, pi.y - v[u].y, pi.z - v[u].z);
+            mask |= r2 < h2 ? (1u << (k0 + u)) : 0u;
         }
     }
-    return mask;
+    mask &= len >= 32u ? 0xffffffffu : (1u << len) - 1u;
+    const uint32_t self = s - c0;  // wraps to a large value when s < c0
+    return self < 32u ? mask & ~(1u << self) : mask;
 }
 
 // Slot ranges of the 9 stencil rows (z-major, then y), fetched together.
