@@ -1,6 +1,6 @@
 """Benchmark of the PBF step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-adjoint]
 
 Workload: the 1M-particle dam break (h = 0.02, box 2.027 x 2.027 x 1.0135,
 102 x 102 x 51 = 530,604 cells, 4 solver iterations, dt = 1/120), seeded
@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the forward+adjoint leg and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (reference kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -207,6 +209,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(torch, reps=2)), flush=True)
+        return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -247,7 +252,11 @@ def main():
     t_prof = (time.perf_counter() - t0) / args.steps
     eng.set_profiling(False)
 
-    # same K steps without events or host waits between steps
+    # the same K steps (same initial state and warm-up) without events or
+    # host waits between steps
+    eng.upload_state(x, v)
+    for _ in range(args.warmup):
+        eng.step(DT, stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
