@@ -98,8 +98,11 @@ typedef enum mtlp_buffer {
     MTLP_BUF_SORTED_INDEX = 5,    /* uint32[n]                              */
     MTLP_BUF_NEIGHBOR_COUNTS = 6, /* uint32[n]                              */
     MTLP_BUF_LAMBDAS = 7,         /* float[n], last iteration, orig. order  */
-    MTLP_BUF_SORTED_POSITIONS = 8 /* float4[n]; .w of a step's final iterate
+    MTLP_BUF_SORTED_POSITIONS = 8, /* float4[n]; .w of a step's final iterate
                                      holds that iterate's lambda input     */
+    MTLP_BUF_FOAM_INTENSITY = 9   /* float[n], computeFoam of the last step
+                                     (the reference's private _foamIntensity,
+                                     Engine.hpp:177)                       */
 } mtlp_buffer;
 
 /* Largest scan length (Engine.hpp:93 caps the reference at 1024^2; the
@@ -147,9 +150,26 @@ mtlp_status mtlp_set_state_device(mtlp_engine* e, mtlp_stream stream, const floa
 mtlp_status mtlp_get_state_device(mtlp_engine* e, mtlp_stream stream, float* d_x4, float* d_v4);
 
 /* Engine::step(cmd, dt) (Engine.cpp:685-811): predict, spatial hash,
- * iterations x (density/lambda, delta), scatter back, finalize.  dt <= 0 is
- * a no-op returning MTLP_OK. */
+ * iterations x (density/lambda, delta[, sphere collision]), scatter back,
+ * finalize, foam.  dt <= 0 is a no-op returning MTLP_OK. */
 mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt);
+/* Engine::applyMousePulse(cmd, rayOrigin, rayDir, force, radius)
+ * (Engine.hpp:61-65, Engine.cpp:813-841, Constraints.metal:222-253): adds
+ * force * exp(-d^2 / (0.4 radius^2)) to the velocity of every particle in
+ * front of the ray origin within `radius` of the ray (ray_dir normalised). */
+mtlp_status mtlp_apply_mouse_pulse(mtlp_engine* e, mtlp_stream stream, const float ray_origin[3],
+                                   const float ray_dir[3], const float force[3], float radius);
+/* The sphere-drop scene's kinematic sphere (Engine.hpp:180-184).  mtlp_reset
+ * with MTLP_SCENE_SPHERE_DROP switches it on at rest at (0, 4.5, 0), radius
+ * 0.6 (Engine.cpp:522-527); each mtlp_step then integrates it on the host
+ * (Engine.cpp:693-712) and pushes solver iterates out of it after every
+ * delta pass (sphereCollision, Constraints.metal:262-277).  Steps with the
+ * sphere on are not replayed from a HIP graph, and the rollout adjoint
+ * rejects them.  NULL arguments leave that part unchanged. */
+mtlp_status mtlp_set_sphere(mtlp_engine* e, int enabled, const float center[3], const float velocity[3],
+                            float radius);
+mtlp_status mtlp_get_sphere(const mtlp_engine* e, int* enabled, float center[3], float velocity[3],
+                            float* radius);
 /* Engine::buildSpatialHash(cmd) (Engine.cpp:601-679) on the current
  * positions: hash, count, scan, stable scatter, gather, neighbour count. */
 mtlp_status mtlp_build_spatial_hash(mtlp_engine* e, mtlp_stream stream);

@@ -29,6 +29,7 @@ BUF_SORTED_INDEX = 5
 BUF_NEIGHBOR_COUNTS = 6
 BUF_LAMBDAS = 7
 BUF_SORTED_POSITIONS = 8
+BUF_FOAM_INTENSITY = 9
 
 TAPE_POSITIONS = 0
 TAPE_VELOCITIES = 1
@@ -119,6 +120,9 @@ SIGNATURES = {
     "mtlp_autotune": (_I, [_P, _P, _F, ctypes.c_char_p, ctypes.POINTER(_U32)]),
     "mtlp_set_stencil_block": (_I, [_P, _U32]),
     "mtlp_set_neighbor_lists": (_I, [_P, _I]),
+    "mtlp_apply_mouse_pulse": (_I, [_P, _P, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F), _F]),
+    "mtlp_set_sphere": (_I, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F]),
+    "mtlp_get_sphere": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)]),
     "mtlp_set_state_device": (_I, [_P, _P, _P, _P]),
     "mtlp_get_state_device": (_I, [_P, _P, _P, _P]),
     "mtlp_rollout": (_I, [_P, _P, _F, _U32, _U32]),

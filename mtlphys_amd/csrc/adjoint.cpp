@@ -15,6 +15,11 @@ mtlp_status check_single(const mtlp_engine* e) {
     return MTLP_OK;
 }
 
+mtlp_status check_no_sphere(const mtlp_engine* e) {
+    if (e->sphere.on) return fail(MTLP_E_INVALID, "the kinematic sphere's collision is not differentiated");
+    return MTLP_OK;
+}
+
 mtlp_status check_dt(float dt) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MTLP_E_INVALID, "dt must be finite and > 0");
     return MTLP_OK;
@@ -150,6 +155,7 @@ extern "C" {
 
 mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps, uint32_t ckpt_every) {
     MTLP_TRY(check_single(e));
+    MTLP_TRY(check_no_sphere(e));
     MTLP_TRY(check_dt(dt));
     if (steps == 0 || ckpt_every == 0) return fail(MTLP_E_INVALID, "steps and ckpt_every must be >= 1");
     DeviceGuard g(e->device);
@@ -279,6 +285,7 @@ mtlp_status mtlp_density_vjp(mtlp_engine* e, mtlp_stream stream, const float* d_
 
 mtlp_status mtlp_tape_record(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps) {
     MTLP_TRY(check_single(e));
+    MTLP_TRY(check_no_sphere(e));
     MTLP_TRY(check_dt(dt));
     if (steps == 0) return fail(MTLP_E_INVALID, "steps must be >= 1");
     DeviceGuard g(e->device);

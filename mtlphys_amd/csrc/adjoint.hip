@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kB) void k_apply_delta_tape(const float4* __restric
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        apply_delta_slot<true, true>(s, sc, cur, cell_start, a.n, next, rr, nl);
+        apply_delta_slot<true, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
     });
 }
 

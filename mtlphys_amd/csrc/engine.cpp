@@ -121,6 +121,7 @@ FinalDev make_final(const mtlp_params& p) {
     f.floor_f = p.floor_friction;
     f.wall_f = p.wall_friction;
     f.vmax_h = p.vmax * p.h;  // Constraints.metal:347: 1.0f * h * invDt
+    f.foam_max_nbr = 96.0f;   // kMaxNeighborsForViz, Engine.cpp:27
     return f;
 }
 
@@ -204,6 +205,7 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->tickets, na));
     HIP_TRY(dev_alloc(&e->nbr, na));
     HIP_TRY(dev_alloc(&e->lambdas, na));
+    HIP_TRY(dev_alloc(&e->foam, na));
     HIP_TRY(dev_alloc(&e->tmp_index, na));
     HIP_TRY(dev_alloc(&e->sorted_index, na));
     HIP_TRY(dev_alloc(&e->sorted[0], na + kChunkSlack));  // the stencil filter reads past the end
@@ -217,6 +219,7 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(hipMemset(e->keys, 0, sizeof(uint32_t) * na));
     HIP_TRY(hipMemset(e->nbr, 0, sizeof(uint32_t) * na));
     HIP_TRY(hipMemset(e->lambdas, 0, sizeof(float) * na));
+    HIP_TRY(hipMemset(e->foam, 0, sizeof(float) * na));
     HIP_TRY(hipMemset(e->sorted_index, 0, sizeof(uint32_t) * na));
     HIP_TRY(hipMemset(e->sorted[0], 0, sizeof(float4) * (na + kChunkSlack)));
     HIP_TRY(hipMemset(e->sorted[1], 0, sizeof(float4) * (na + kChunkSlack)));
@@ -321,10 +324,28 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
         HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
         clk.mark("count_neighbors");
     }
-    HIP_TRY(launch_finalize(e->sorted[cur], e->sorted_index, a, e->x, e->v, e->prev, s));
+    HIP_TRY(launch_finalize(e->sorted[cur], e->sorted_index, a, e->x, e->v, e->prev, e->nbr, e->foam, s));
     clk.mark("scatter_finalize");
     e->final_buf = cur;
     return MTLP_OK;
+}
+
+// Sphere kinematics of the sphere-drop scene (Engine.cpp:693-712): gravity,
+// 0.998 drag per step, walls with restitution 0.4 (the scene's bounds).
+void advance_sphere(mtlp_engine::Sphere& sp, const mtlp_params& p, float dt) {
+    sp.v[1] += -9.81f * dt;
+    for (int a = 0; a < 3; ++a) sp.v[a] *= 0.998f;
+    for (int a = 0; a < 3; ++a) sp.c[a] += sp.v[a] * dt;
+    for (int a = 0; a < 3; ++a) {
+        if (sp.c[a] - sp.r < p.bounds_min[a]) {
+            sp.c[a] = p.bounds_min[a] + sp.r;
+            sp.v[a] = -sp.v[a] * 0.4f;
+        }
+        if (sp.c[a] + sp.r > p.bounds_max[a]) {
+            sp.c[a] = p.bounds_max[a] - sp.r;
+            sp.v[a] = -sp.v[a] * 0.4f;
+        }
+    }
 }
 
 mtlp_status capture_graph(mtlp_engine* e, float dt) {
@@ -467,6 +488,9 @@ mtlp_status mtlp_reset(mtlp_engine* e, uint32_t n, uint32_t scene, uint64_t seed
         e->scene_counts = {n};
         MTLP_TRY(allocate_all(e));
     }
+    // the sphere-drop scene's sphere starts at rest above the lake (Engine.cpp:522-527)
+    e->sphere = mtlp_engine::Sphere{};
+    e->sphere.on = !e->batch && scene == MTLP_SCENE_SPHERE_DROP;
     return seed_scenes(e, scene, seed);
 }
 
@@ -572,6 +596,10 @@ mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt) {
     if (e->n == 0) return MTLP_OK;
     DeviceGuard g(e->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (e->sphere.on) {
+        advance_sphere(e->sphere, e->scene_params[0], dt);
+        return enqueue_step(e, s, dt);  // the sphere moves every step: no graph replay
+    }
     if (e->use_graph) {
         if (!e->graph_exec || e->graph_dt != dt) MTLP_TRY(capture_graph(e, dt));
         HIP_TRY(hipGraphLaunch(e->graph_exec, s));
@@ -579,6 +607,43 @@ mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt) {
         return MTLP_OK;
     }
     return enqueue_step(e, s, dt);
+}
+
+mtlp_status mtlp_apply_mouse_pulse(mtlp_engine* e, mtlp_stream stream, const float ray_origin[3],
+                                   const float ray_dir[3], const float force[3], float radius) {
+    MTLP_TRY(check_engine(e));
+    if (!ray_origin || !ray_dir || !force) return fail(MTLP_E_INVALID, "NULL argument");
+    if (e->n == 0) return MTLP_OK;
+    DeviceGuard g(e->device);
+    HIP_TRY(launch_mouse_pulse(e->x, e->v, e->n, ray_origin, ray_dir, force, radius,
+                               reinterpret_cast<hipStream_t>(stream)));
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_set_sphere(mtlp_engine* e, int enabled, const float center[3], const float velocity[3],
+                            float radius) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (enabled && e->batch) return fail(MTLP_E_INVALID, "the kinematic sphere is for single-scene engines");
+    if (enabled && !(radius > 0.0f)) return fail(MTLP_E_INVALID, "sphere radius must be > 0");
+    e->sphere.on = enabled != 0;
+    for (int a = 0; a < 3; ++a) {
+        if (center) e->sphere.c[a] = center[a];
+        if (velocity) e->sphere.v[a] = velocity[a];
+    }
+    if (radius > 0.0f) e->sphere.r = radius;
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_get_sphere(const mtlp_engine* e, int* enabled, float center[3], float velocity[3],
+                            float* radius) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (enabled) *enabled = e->sphere.on ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        if (center) center[a] = e->sphere.c[a];
+        if (velocity) velocity[a] = e->sphere.v[a];
+    }
+    if (radius) *radius = e->sphere.r;
+    return MTLP_OK;
 }
 
 mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n_steps) {
@@ -754,6 +819,7 @@ mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, siz
         case MTLP_BUF_NEIGHBOR_COUNTS: p = e->nbr; b = n * sizeof(uint32_t); break;
         case MTLP_BUF_LAMBDAS: p = e->lambdas; b = n * sizeof(float); break;
         case MTLP_BUF_SORTED_POSITIONS: p = e->sorted[e->final_buf]; b = n * sizeof(float4); break;
+        case MTLP_BUF_FOAM_INTENSITY: p = e->foam; b = n * sizeof(float); break;
         default: return fail(MTLP_E_INVALID, "unknown buffer id");
     }
     *d_ptr = p;

@@ -156,6 +156,7 @@ struct mtlp_engine {
     uint32_t* tickets = nullptr;
     uint32_t* nbr = nullptr;
     float* lambdas = nullptr;
+    float* foam = nullptr;
     // particle buffers (sorted order)
     uint32_t* tmp_index = nullptr;
     uint32_t* sorted_index = nullptr;
@@ -198,7 +199,7 @@ struct mtlp_engine {
     }
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
-        dev_free(nbr); dev_free(lambdas); dev_free(tmp_index); dev_free(sorted_index);
+        dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
         dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list); dev_free(nlist); dev_free(ncount);
         n_alloc = 0;
     }
@@ -231,6 +232,15 @@ struct mtlp_engine {
         return use_lists ? NbrList{nlist, ncount, n_alloc} : NbrList{nullptr, nullptr, 0u};
     }
 
+    // Kinematic sphere of the sphere-drop scene (Engine.hpp:180-184),
+    // integrated on the host each step like Engine.cpp:693-712.
+    struct Sphere {
+        bool on = false;
+        float c[3] = {0.0f, 4.5f, 0.0f};
+        float v[3] = {0.0f, 0.0f, 0.0f};
+        float r = 0.6f;
+    } sphere;
+
     StepArgs step_args(float dt, bool predict) const {
         StepArgs a{};
         a.scenes = d_scenes;
@@ -239,6 +249,7 @@ struct mtlp_engine {
         a.n = n;
         a.dt = dt;
         a.predict = predict ? 1u : 0u;
+        a.sphere = SphereDev{sphere.c[0], sphere.c[1], sphere.c[2], sphere.r, sphere.on ? 1u : 0u};
         return a;
     }
 };

@@ -62,6 +62,7 @@ struct FinalDev {
     float bminx, bminy, bminz, bmaxx, bmaxy, bmaxz;
     float damping, weps, floor_f, wall_f;
     float vmax_h;         // vmax * h (the cap is vmax_h / dt, Constraints.metal:347)
+    float foam_max_nbr;   // neighbour count of zero surface-ness (Engine.cpp:27)
 };
 
 struct SceneDev {
@@ -73,6 +74,13 @@ struct SceneDev {
 };
 
 // Step-wide inputs shared by every scene.
+// Kinematic sphere of the sphere-drop scene (SphereParams, Shared.h:78-81):
+// solver iterates inside it are pushed to its surface.
+struct SphereDev {
+    float cx, cy, cz, r;
+    uint32_t on;
+};
+
 struct StepArgs {
     const SceneDev* scenes;
     const uint32_t* p_off;  // n_scenes + 1 particle offsets
@@ -80,6 +88,7 @@ struct StepArgs {
     uint32_t n;             // total particles
     float dt;
     uint32_t predict;       // 0: hash positions as they are (buildSpatialHash)
+    SphereDev sphere;       // single-scene engines only
 };
 
 // Buffers of one spatial-hash build.
@@ -143,8 +152,14 @@ hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
+// Scatter back + finalize; with foam non-NULL also computeFoam from the
+// step's neighbour counts (nbr, original order).
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
-                           const StepArgs& a, float4* x, float4* v, float4* prev, hipStream_t s);
+                           const StepArgs& a, float4* x, float4* v, float4* prev,
+                           const uint32_t* nbr, float* foam, hipStream_t s);
+// applyMousePulse (Constraints.metal:222-253) on n particles.
+hipError_t launch_mouse_pulse(const float4* x, float4* v, uint32_t n, const float origin[3],
+                              const float dir[3], const float force[3], float radius, hipStream_t s);
 
 // ---- adjoint (adjoint.hip) ---------------------------------------------
 // What the adjoint of one step reads from its taped forward: the hash, the

@@ -267,15 +267,26 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
 }
 
 // applyDeltaSorted for slot s: next[s] = clamp(p_s + cap(R_s)) with
-// R_s = sum_j (lambda_s + lambda_j + s_corr) grad W / rho0.  With kTape R_s
-// (before the 0.3 h cap) goes to rr[s] for the adjoint.  With kList the
-// neighbours come from the density pass's list.
+// R_s = sum_j (lambda_s + lambda_j + s_corr) grad W / rho0, then the
+// sphere-drop scene's sphereCollision on next[s] (Engine.cpp:760-771).
+// With kTape R_s (before the 0.3 h cap) goes to rr[s] for the adjoint.  With
+// kList the neighbours come from the density pass's list.
+// sphereCollision (Constraints.metal:262-277): a point inside the sphere
+// moves to the surface along the ray from the centre.
+__device__ __forceinline__ float4 sphere_push(float4 p, const SphereDev& sp) {
+    const float dx = p.x - sp.cx, dy = p.y - sp.cy, dz = p.z - sp.cz;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < sp.r * sp.r)) return p;
+    const float r = sqrtf(fmaxf(r2, 1e-8f));
+    return make_float4(sp.cx + dx / r * sp.r, sp.cy + dy / r * sp.r, sp.cz + dz / r * sp.r, 1.0f);
+}
+
 template <bool kTape, bool kList>
 __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
                                                  const float4* __restrict__ cur,
                                                  const uint32_t* __restrict__ cell_start, uint32_t n,
                                                  float4* __restrict__ next, float4* __restrict__ rr,
-                                                 const NbrList& nl) {
+                                                 const NbrList& nl, const SphereDev& sphere) {
     const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     const double lam_d = pi.w;
@@ -297,19 +308,23 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
         dz *= f;
     }
     // clamp to the box (Constraints.metal:197), one rounding to FP32
-    next[s] = make_float4((float)fmin(fmax((double)pi.x + dx, (double)sd.bminx), (double)sd.bmaxx),
-                          (float)fmin(fmax((double)pi.y + dy, (double)sd.bminy), (double)sd.bmaxy),
-                          (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
+    float4 q = make_float4((float)fmin(fmax((double)pi.x + dx, (double)sd.bminx), (double)sd.bmaxx),
+                           (float)fmin(fmax((double)pi.y + dy, (double)sd.bminy), (double)sd.bmaxy),
+                           (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
+    if (sphere.on) q = sphere_push(q, sphere);
+    next[s] = q;
 }
 
 // Scatter back + finalize for slot s, particle i = sorted_index[s]: x_in[i]
 // is the position at the start of the step; the step's result goes to
 // x_out[i], v_out[i] (x_out may be x_in), and x_in[i] to prev[i] if given.
+// With foam, also the foam intensity from nbr[i] and the new velocity.
 __device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, float dt,
                                               const float4* __restrict__ fin,
                                               const uint32_t* __restrict__ sorted_index,
                                               const float4* x_in, float4* x_out, float4* __restrict__ v_out,
-                                              float4* __restrict__ prev) {
+                                              float4* __restrict__ prev, const uint32_t* __restrict__ nbr = nullptr,
+                                              float* __restrict__ foam = nullptr) {
     const float inv_dt = 1.0f / dt;       // Engine.cpp:561
     const float vmax = fd.vmax_h * inv_dt;  // Constraints.metal:347
     const uint32_t i = sorted_index[s];
@@ -348,6 +363,14 @@ __device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, fl
     }
     x_out[i] = make_float4(px, py, pz, 1.0f);
     v_out[i] = make_float4(vx, vy, vz, 0.0f);
+    if (foam) {
+        // computeFoam (Constraints.metal:287-299): surface-ness from the
+        // neighbour count times smoothstep(1.2, 4.5, |v|)
+        const float speed = sqrtf(vx * vx + vy * vy + vz * vz);
+        const float surface = 1.0f - fminf(fmaxf((float)nbr[i] / fd.foam_max_nbr, 0.0f), 1.0f);
+        const float t = fminf(fmaxf((speed - 1.2f) / (4.5f - 1.2f), 0.0f), 1.0f);
+        foam[i] = surface * (t * t * (3.0f - 2.0f * t));
+    }
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }

@@ -35,18 +35,45 @@ __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ c
     const uint32_t n = a.n;
     if (s >= n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
-        apply_delta_slot<false, kList>(s, sc, cur, cell_start, n, next, nullptr, nl);
+        apply_delta_slot<false, kList>(s, sc, cur, cell_start, n, next, nullptr, nl, a.sphere);
     });
 }
 
-// Scatter back + finalize: slot s writes particle sorted_index[s].  The old
-// position (still in x) becomes prev; x and v get the step's result.
+// Scatter back + finalize (+ computeFoam): slot s writes particle
+// sorted_index[s].  The old position (still in x) becomes prev; x and v get
+// the step's result.
 __global__ __launch_bounds__(kBlock) void k_finalize(
     const float4* __restrict__ fin, const uint32_t* __restrict__ sorted_index, StepArgs a,
-    float4* x, float4* __restrict__ v, float4* __restrict__ prev) {
+    float4* x, float4* __restrict__ v, float4* __restrict__ prev, const uint32_t* __restrict__ nbr,
+    float* __restrict__ foam) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= a.n) return;
-    with_scene(s, a, [&](const SceneDev& sc) { finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, x, v, prev); });
+    with_scene(s, a, [&](const SceneDev& sc) {
+        finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, x, v, prev, nbr, foam);
+    });
+}
+
+// applyMousePulse (Constraints.metal:222-253): velocity impulse with a
+// Gaussian falloff around the cursor ray, for particles in front of its
+// origin and within `radius` of it.
+__global__ __launch_bounds__(kBlock) void k_mouse_pulse(const float4* __restrict__ x, float4* __restrict__ v,
+                                                       uint32_t n, float ox, float oy, float oz, float dx,
+                                                       float dy, float dz, float fx, float fy, float fz,
+                                                       float radius) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = x[i];
+    const float tx = p.x - ox, ty = p.y - oy, tz = p.z - oz;
+    const float t = tx * dx + ty * dy + tz * dz;
+    if (t <= 0.0f) return;
+    const float qx = p.x - (ox + dx * t), qy = p.y - (oy + dy * t), qz = p.z - (oz + dz * t);
+    const float r2 = qx * qx + qy * qy + qz * qz;
+    const float R2 = radius * radius;
+    if (r2 < R2) {
+        const float w = expf(-r2 / (R2 * 0.4f));
+        const float4 vi = v[i];
+        v[i] = make_float4(vi.x + fx * w, vi.y + fy * w, vi.z + fz * w, 0.0f);
+    }
 }
 
 template <uint32_t kB, bool kList>
@@ -108,10 +135,19 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
 }
 
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
-                           const StepArgs& a, float4* x, float4* v, float4* prev, hipStream_t s) {
+                           const StepArgs& a, float4* x, float4* v, float4* prev,
+                           const uint32_t* nbr, float* foam, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, final_sorted,
-                       sorted_index, a, x, v, prev);
+                       sorted_index, a, x, v, prev, nbr, foam);
+    return hipGetLastError();
+}
+
+hipError_t launch_mouse_pulse(const float4* x, float4* v, uint32_t n, const float origin[3],
+                              const float dir[3], const float force[3], float radius, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mouse_pulse, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, x, v, n, origin[0],
+                       origin[1], origin[2], dir[0], dir[1], dir[2], force[0], force[1], force[2], radius);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ _BUFFER_LAYOUT = {
     C.BUF_NEIGHBOR_COUNTS: (np.uint32, 1),
     C.BUF_LAMBDAS: (np.float32, 1),
     C.BUF_SORTED_POSITIONS: (np.float32, 4),
+    C.BUF_FOAM_INTENSITY: (np.float32, 1),
 }
 
 
@@ -138,6 +139,25 @@ class Engine:
         """Engine::step(cmd, dt) (Engine.cpp:685-811); asynchronous."""
         C.check(C.lib().mtlp_step(self._h, _stream_handle(stream), ctypes.c_float(dt)), "mtlp_step")
 
+    def apply_mouse_pulse(self, ray_origin, ray_dir, force, radius, stream=None):
+        """Engine::applyMousePulse (Engine.cpp:813-841); asynchronous."""
+        f3 = ctypes.c_float * 3
+        C.check(C.lib().mtlp_apply_mouse_pulse(self._h, _stream_handle(stream), f3(*ray_origin), f3(*ray_dir),
+                                               f3(*force), ctypes.c_float(radius)), "mtlp_apply_mouse_pulse")
+
+    def set_sphere(self, enabled=True, center=None, velocity=None, radius=0.0):
+        f3 = ctypes.c_float * 3
+        C.check(C.lib().mtlp_set_sphere(self._h, 1 if enabled else 0, None if center is None else f3(*center),
+                                        None if velocity is None else f3(*velocity), ctypes.c_float(radius)),
+                "mtlp_set_sphere")
+
+    def sphere(self):
+        """(enabled, center, velocity, radius) of the kinematic sphere."""
+        on = ctypes.c_int()
+        c, v, r = (ctypes.c_float * 3)(), (ctypes.c_float * 3)(), ctypes.c_float()
+        C.check(C.lib().mtlp_get_sphere(self._h, ctypes.byref(on), c, v, ctypes.byref(r)), "mtlp_get_sphere")
+        return bool(on.value), list(c), list(v), r.value
+
     def build_spatial_hash(self, stream=None):
         """Engine::buildSpatialHash(cmd) (Engine.cpp:601-679); asynchronous."""
         C.check(C.lib().mtlp_build_spatial_hash(self._h, _stream_handle(stream)),
@@ -213,6 +233,9 @@ class Engine:
 
     def sorted_positions_buffer(self):
         return self.buffer(C.BUF_SORTED_POSITIONS)
+
+    def foam_intensity_buffer(self):
+        return self.buffer(C.BUF_FOAM_INTENSITY)
 
     # ---- diagnostics ------------------------------------------------------
     def set_profiling(self, on=True):

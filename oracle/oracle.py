@@ -32,6 +32,8 @@ class OrcParams(ctypes.Structure):
         ("vmax", ctypes.c_float),
         ("scorr_n", ctypes.c_uint32),
         ("iterations", ctypes.c_uint32),
+        ("sphere", ctypes.c_float * 4),
+        ("sphere_on", ctypes.c_uint32),
     ]
 
 
@@ -102,6 +104,9 @@ def lib():
             "orc_step_kinks_d": (None, [ctypes.POINTER(OrcTape), u32, PP, ctypes.c_float, vp]),
             "orc_density_at_d": (None, [vp, u32, PP, vp]),
             "orc_density_vjp_d": (None, [vp, u32, PP, vp, vp]),
+            "orc_sphere_advance": (None, [vp, PP, ctypes.c_float]),
+            "orc_foam_f64": (None, [vp, vp, u32, ctypes.c_double, vp]),
+            "orc_mouse_pulse_f64": (None, [vp, vp, u32, vp, vp, vp, ctypes.c_double]),
             "orc_set_threads": (None, [ctypes.c_int]),
             "orc_get_threads": (ctypes.c_int, []),
         }
@@ -127,7 +132,7 @@ def params(src=None, **overrides):
         if isinstance(src, dict):
             items.update(src)
         else:
-            items.update({name: getattr(src, name) for name, _ in OrcParams._fields_})
+            items.update({name: getattr(src, name) for name, _ in OrcParams._fields_ if hasattr(src, name)})
     items.update(overrides)
     for k, v in items.items():
         if hasattr(v, "__len__"):
@@ -398,3 +403,28 @@ def rollout_grad(x0, v0, p, dt, steps, target, w_density=0.1, exact_sums=True):
         xb, vb, g1 = step_vjp_d(t, p, dt, xb, vb, sums)
         gb += g1
     return L, xb, vb, gb
+
+
+# ---- scene extras -------------------------------------------------------------
+
+def sphere_advance(state, p, dt):
+    """One step of the sphere kinematics; state = [cx, cy, cz, vx, vy, vz] (float32)."""
+    st = np.ascontiguousarray(state, np.float32).copy()
+    lib().orc_sphere_advance(_p(st), ctypes.byref(p), ctypes.c_float(dt))
+    return st
+
+
+def foam(v4, nbr, max_nbr=96.0):
+    v4 = f64x4(v4)
+    out = np.empty(len(v4), np.float64)
+    lib().orc_foam_f64(_p(v4), _p(np.ascontiguousarray(nbr, np.uint32)), len(v4), ctypes.c_double(max_nbr), _p(out))
+    return out
+
+
+def mouse_pulse(x4, v4, origin, direction, force, radius):
+    x4 = f64x4(x4)
+    v = np.array(v4, np.float64, copy=True, order="C")
+    d3 = lambda a: np.ascontiguousarray(a, np.float64)
+    lib().orc_mouse_pulse_f64(_p(x4), _p(v), len(x4), _p(d3(origin)), _p(d3(direction)), _p(d3(force)),
+                              ctypes.c_double(radius))
+    return v

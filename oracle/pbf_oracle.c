@@ -188,6 +188,59 @@ void orc_build_spatial_hash(const float* p4, uint32_t n, const orc_grid* g,
     }
 }
 
+void orc_sphere_advance(float state[6], const orc_params* p, float dt)
+{
+    /* Engine.cpp:693-712 (bounds: the scene's, kBoundsMin/Max for scene 2) */
+    float* c = state;
+    float* v = state + 3;
+    const float r = p->sphere[3];
+    v[1] += -9.81f * dt;
+    for (int a = 0; a < 3; ++a) v[a] *= 0.998f;
+    for (int a = 0; a < 3; ++a) c[a] += v[a] * dt;
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] - r < p->bounds_min[a]) { c[a] = p->bounds_min[a] + r; v[a] = -v[a] * 0.4f; }
+        if (c[a] + r > p->bounds_max[a]) { c[a] = p->bounds_max[a] - r; v[a] = -v[a] * 0.4f; }
+    }
+}
+
+void orc_foam_f64(const double* v4, const uint32_t* nbr, uint32_t n, double max_nbr, double* foam)
+{
+    /* Constraints.metal:287-299: (1 - clamp(nbr / max, 0, 1)) * smoothstep(1.2, 4.5, |v|),
+     * smoothstep(e0, e1, x) = t^2 (3 - 2t), t = clamp((x - e0) / (e1 - e0), 0, 1) */
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* v = v4 + 4 * (size_t)i;
+        const double speed = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        double surf = (double)nbr[i] / max_nbr;
+        surf = 1.0 - (surf < 0 ? 0 : (surf > 1 ? 1 : surf));
+        double t = (speed - 1.2) / (4.5 - 1.2);
+        t = t < 0 ? 0 : (t > 1 ? 1 : t);
+        foam[i] = surf * t * t * (3.0 - 2.0 * t);
+    }
+}
+
+void orc_mouse_pulse_f64(const double* x4, double* v4, uint32_t n, const double origin[3],
+                         const double dir[3], const double force[3], double radius)
+{
+    /* Constraints.metal:231-253 */
+    const double R2 = radius * radius;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double* x = x4 + 4 * (size_t)i;
+        const double to[3] = { x[0] - origin[0], x[1] - origin[1], x[2] - origin[2] };
+        const double t = to[0] * dir[0] + to[1] * dir[1] + to[2] * dir[2];
+        if (t <= 0.0) continue;
+        double r2 = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double d = x[a] - (origin[a] + dir[a] * t);
+            r2 += d * d;
+        }
+        if (r2 < R2) {
+            const double w = exp(-r2 / (R2 * 0.4));
+            for (int a = 0; a < 3; ++a) v4[4 * (size_t)i + a] += force[a] * w;
+            v4[4 * (size_t)i + 3] = 0.0;
+        }
+    }
+}
+
 void orc_set_threads(int threads)
 {
 #ifdef _OPENMP

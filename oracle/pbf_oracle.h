@@ -44,6 +44,10 @@
  *                     sum_i dx_i = 0; step clamp |dx| = max_step
  *   predict/finalize  free fall (test_pbf.cpp:47-65), per-wall hand cases,
  *                     CFL cap
+ *   sphere, foam,     hand cases: a point inside the sphere lands on its
+ *   mouse pulse       surface along the centre ray; foam 0 below 1.2 m/s and
+ *                     for >= 96 neighbours, smoothstep values between; pulse
+ *                     weight 1 on the ray, exp(-2.5) at the rim, 0 behind
  */
 #ifndef PBF_ORACLE_H
 #define PBF_ORACLE_H
@@ -83,6 +87,10 @@ typedef struct orc_params {
     float    vmax;           /* speed cap as a fraction of h/dt              */
     uint32_t scorr_n;
     uint32_t iterations;
+    /* sphere-drop scene (SphereParams, Shared.h:78-81): centre xyz, radius;
+     * when sphere_on, every delta pass ends with sphereCollision */
+    float    sphere[4];
+    uint32_t sphere_on;
 } orc_params;
 
 void orc_default_params(orc_params* p);
@@ -189,6 +197,17 @@ void orc_step_kinks_d(const orc_tape* tape, uint32_t n, const orc_params* p, flo
 void orc_density_at_d(const double* x4, uint32_t n, const orc_params* p, double* rho);
 void orc_density_vjp_d(const double* x4, uint32_t n, const orc_params* p,
                        const double* rhobar, double* xbar);
+
+/* ---- scene extras ------------------------------------------------------ */
+/* Sphere kinematics of one step (Engine.cpp:693-712), FP32 like the
+ * reference: state = centre xyz, velocity xyz; radius and bounds from p. */
+void orc_sphere_advance(float state[6], const orc_params* p, float dt);
+/* computeFoam (Constraints.metal:287-299) from velocities and neighbour
+ * counts, original order; max_nbr = kMaxNeighborsForViz (Engine.cpp:27). */
+void orc_foam_f64(const double* v4, const uint32_t* nbr, uint32_t n, double max_nbr, double* foam);
+/* applyMousePulse (Constraints.metal:222-253) on v4, in place. */
+void orc_mouse_pulse_f64(const double* x4, double* v4, uint32_t n, const double origin[3],
+                         const double dir[3], const double force[3], double radius);
 
 /* OpenMP thread count for the per-particle stages (0 = library default). */
 void orc_set_threads(int threads);

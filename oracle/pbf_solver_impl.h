@@ -142,7 +142,8 @@ void CAT(orc_density_lambda_, SFX)(const REAL* sx4, const uint32_t* cell_start,
     }
 }
 
-/* applyDeltaSorted, Constraints.metal:140-199. */
+/* applyDeltaSorted, Constraints.metal:140-199, then sphereCollision when
+ * the sphere is on (Engine.cpp:760-771). */
 void CAT(orc_apply_delta_, SFX)(const REAL* sx4, const REAL* lambda_sorted,
                                const uint32_t* cell_start, uint32_t n,
                                const orc_grid* g, const orc_params* p,
@@ -188,6 +189,16 @@ void CAT(orc_apply_delta_, SFX)(const REAL* sx4, const REAL* lambda_sorted,
         REAL* out = sx4_next + 4 * (size_t)s;
         for (int a = 0; a < 3; ++a) out[a] = FN(clampr)(pi[a] + d[a], k.bmin[a], k.bmax[a]);
         out[3] = (REAL)1;
+        if (p->sphere_on) {
+            /* sphereCollision (Constraints.metal:262-277) on the new iterate */
+            REAL q[3], r2 = 0;
+            for (int a = 0; a < 3; ++a) { q[a] = out[a] - (REAL)p->sphere[a]; r2 += q[a] * q[a]; }
+            const REAL rr = (REAL)p->sphere[3] * (REAL)p->sphere[3];
+            if (r2 < rr) {
+                const REAL r = SQRT_OF(r2 > (REAL)1e-8f ? r2 : (REAL)1e-8f);
+                for (int a = 0; a < 3; ++a) out[a] = (REAL)p->sphere[a] + q[a] / r * (REAL)p->sphere[3];
+            }
+        }
     }
 }
 
