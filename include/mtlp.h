@@ -245,6 +245,19 @@ mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
  * candidates again.  Results are bit-identical with lists on or off. */
 mtlp_status mtlp_set_neighbor_lists(mtlp_engine* e, int enable);
 
+/* ---- state I/O (no reference counterpart) ------------------------------
+ * A little-endian binary file: "MTLPSTAT", u32 version (1), u32 n_scenes,
+ * u32 n (particles), u32 flags (bit 0: batch), u32 scene id, u32 sphere on,
+ * f32 sphere centre[3], velocity[3], radius; per scene an mtlp_params (92
+ * bytes) and a u32 particle count; then float4 positions[n] and
+ * velocities[n] in original order.  mtlp_save_state synchronises and
+ * writes the engine's state; mtlp_load_state reads it back: a single-scene
+ * engine takes the file's parameters and particle count (reallocating if
+ * they differ), a batch needs the same scene counts. */
+#define MTLP_STATE_VERSION 1u
+mtlp_status mtlp_save_state(mtlp_engine* e, const char* path);
+mtlp_status mtlp_load_state(mtlp_engine* e, const char* path);
+
 /* ---- differentiable rollout (single-scene engines) --------------------
  * The reference has no adjoint; these entries have no counterpart in it.
  * The adjoint differentiates mtlp_step as computed, with the neighbour

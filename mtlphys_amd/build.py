@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmtlp.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-SOURCES = ["engine.cpp", "adjoint.cpp", "scenes.cpp", "scan.hip", "hash.hip", "solver.hip", "adjoint.hip"]
+SOURCES = ["engine.cpp", "adjoint.cpp", "state_io.cpp", "scenes.cpp", "scan.hip", "hash.hip", "solver.hip", "adjoint.hip"]
 HEADERS = ["mtlp_kernels.h", "mtlp_device.h", "mtlp_stencil.h", "mtlp_host.h", "engine_impl.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

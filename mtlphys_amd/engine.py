@@ -105,6 +105,7 @@ class Engine:
         h = ctypes.c_void_p()
         C.check(C.lib().mtlp_create(int(device), ctypes.byref(p), ctypes.byref(h)), "mtlp_create")
         self._h = h
+        self._device = int(device)
         self._scene = C.SCENE_CUBE_DROP
 
     def close(self):
@@ -138,6 +139,14 @@ class Engine:
     def step(self, dt, stream=None):
         """Engine::step(cmd, dt) (Engine.cpp:685-811); asynchronous."""
         C.check(C.lib().mtlp_step(self._h, _stream_handle(stream), ctypes.c_float(dt)), "mtlp_step")
+
+    def save_state(self, path):
+        """Write positions, velocities, parameters and the sphere to `path`."""
+        C.check(C.lib().mtlp_save_state(self._h, str(path).encode()), "mtlp_save_state")
+
+    def load_state(self, path):
+        """Read a state file written by save_state or mtlphys_amd.state.write_state."""
+        C.check(C.lib().mtlp_load_state(self._h, str(path).encode()), "mtlp_load_state")
 
     def apply_mouse_pulse(self, ray_origin, ray_dir, force, radius, stream=None):
         """Engine::applyMousePulse (Engine.cpp:813-841); asynchronous."""
@@ -206,6 +215,20 @@ class Engine:
 
     def buffer(self, which):
         return DeviceBuffer(self, which)
+
+    def tensor(self, which):
+        """Zero-copy torch tensor over an engine buffer (DLPack): (n, 4)
+        float32 for positions/velocities, (n,) otherwise.  It aliases the
+        engine's memory, so steps update it in place; it is valid until
+        reset() or close()."""
+        import torch
+        from . import dlpack
+        buf = self.buffer(which)
+        dtype, comps = _BUFFER_LAYOUT[which]
+        count = buf.nbytes // (np.dtype(dtype).itemsize * comps)
+        shape = (count, comps) if comps > 1 else (count,)
+        cap = dlpack.capsule(buf.ptr, shape, dtype, dlpack.kDLROCM, self._device, owner=self)
+        return torch.utils.dlpack.from_dlpack(cap)
 
     def positions_buffer(self):
         return self.buffer(C.BUF_POSITIONS)
@@ -351,6 +374,7 @@ class Batch(Engine):
         C.check(C.lib().mtlp_batch_create(int(device), len(counts), arr_c, arr_p, ctypes.byref(h)),
                 "mtlp_batch_create")
         self._h = h
+        self._device = int(device)
         self._scene = C.SCENE_EMPTY
         self.counts = counts
         self.scene_params = [p.copy() for p in params]
