@@ -153,6 +153,11 @@ mtlp_status mtlp_get_state_device(mtlp_engine* e, mtlp_stream stream, float* d_x
  * iterations x (density/lambda, delta[, sphere collision]), scatter back,
  * finalize, foam.  dt <= 0 is a no-op returning MTLP_OK. */
 mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt);
+/* Engine::integrate(cmd, dt) (Engine.hpp:69, Engine.cpp:578-595,
+ * Integrate.metal:11-42): the legacy semi-implicit Euler step with
+ * gravity -9.81, damping 0.999 and the box with restitution 0.4; on
+ * positions and velocities in place (no spatial hash). */
+mtlp_status mtlp_integrate(mtlp_engine* e, mtlp_stream stream, float dt);
 /* Engine::applyMousePulse(cmd, rayOrigin, rayDir, force, radius)
  * (Engine.hpp:61-65, Engine.cpp:813-841, Constraints.metal:222-253): adds
  * force * exp(-d^2 / (0.4 radius^2)) to the velocity of every particle in

@@ -122,6 +122,7 @@ SIGNATURES = {
     "mtlp_set_neighbor_lists": (_I, [_P, _I]),
     "mtlp_save_state": (_I, [_P, ctypes.c_char_p]),
     "mtlp_load_state": (_I, [_P, ctypes.c_char_p]),
+    "mtlp_integrate": (_I, [_P, _P, _F]),
     "mtlp_apply_mouse_pulse": (_I, [_P, _P, ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F), _F]),
     "mtlp_set_sphere": (_I, [_P, _I, ctypes.POINTER(_F), ctypes.POINTER(_F), _F]),
     "mtlp_get_sphere": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_F), ctypes.POINTER(_F), ctypes.POINTER(_F)]),

@@ -609,6 +609,17 @@ mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt) {
     return enqueue_step(e, s, dt);
 }
 
+mtlp_status mtlp_integrate(mtlp_engine* e, mtlp_stream stream, float dt) {
+    MTLP_TRY(check_engine(e));
+    if (e->batch) return fail(MTLP_E_INVALID, "integrate runs on single-scene engines");
+    if (e->n == 0) return MTLP_OK;
+    DeviceGuard g(e->device);
+    // Engine.cpp:578-595: fixed gravity and damping, the scene's box
+    HIP_TRY(launch_integrate(e->x, e->v, e->n, dt, -9.81f, 0.999f, make_final(e->scene_params[0]),
+                             reinterpret_cast<hipStream_t>(stream)));
+    return MTLP_OK;
+}
+
 mtlp_status mtlp_apply_mouse_pulse(mtlp_engine* e, mtlp_stream stream, const float ray_origin[3],
                                    const float ray_dir[3], const float force[3], float radius) {
     MTLP_TRY(check_engine(e));

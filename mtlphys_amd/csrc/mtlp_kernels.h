@@ -157,6 +157,9 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
                            const uint32_t* nbr, float* foam, hipStream_t s);
+// integrate (Integrate.metal:11-42) on n particles; bounds from b.
+hipError_t launch_integrate(float4* x, float4* v, uint32_t n, float dt, float gy, float damping,
+                            const FinalDev& b, hipStream_t s);
 // applyMousePulse (Constraints.metal:222-253) on n particles.
 hipError_t launch_mouse_pulse(const float4* x, float4* v, uint32_t n, const float origin[3],
                               const float dir[3], const float force[3], float radius, hipStream_t s);

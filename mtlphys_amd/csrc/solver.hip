@@ -53,6 +53,36 @@ __global__ __launch_bounds__(kBlock) void k_finalize(
     });
 }
 
+// Legacy semi-implicit Euler (Integrate.metal:11-42): v += g dt, v *= damping,
+// x += v dt, then per axis clamp to the box and reflect with restitution 0.4.
+__global__ __launch_bounds__(kBlock) void k_integrate(float4* __restrict__ x, float4* __restrict__ v, uint32_t n,
+                                                     float dt, float gy, float damping, FinalDev b) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p0 = x[i], v0 = v[i];
+    float p[3] = {p0.x, p0.y, p0.z}, w[3] = {v0.x, v0.y + gy * dt, v0.z};
+    const float lo[3] = {b.bminx, b.bminy, b.bminz}, hi[3] = {b.bmaxx, b.bmaxy, b.bmaxz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w[k] *= damping;
+        p[k] += w[k] * dt;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (p[k] < lo[k]) {
+            p[k] = lo[k];
+            w[k] = -w[k] * 0.4f;
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (p[k] > hi[k]) {
+            p[k] = hi[k];
+            w[k] = -w[k] * 0.4f;
+        }
+    x[i] = make_float4(p[0], p[1], p[2], 1.0f);
+    v[i] = make_float4(w[0], w[1], w[2], 0.0f);
+}
+
 // applyMousePulse (Constraints.metal:222-253): velocity impulse with a
 // Gaussian falloff around the cursor ray, for particles in front of its
 // origin and within `radius` of it.
@@ -140,6 +170,14 @@ hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_in
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, final_sorted,
                        sorted_index, a, x, v, prev, nbr, foam);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate(float4* x, float4* v, uint32_t n, float dt, float gy, float damping,
+                            const FinalDev& bounds, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s, x, v, n, dt, gy, damping,
+                       bounds);
     return hipGetLastError();
 }
 

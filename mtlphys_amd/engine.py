@@ -140,6 +140,10 @@ class Engine:
         """Engine::step(cmd, dt) (Engine.cpp:685-811); asynchronous."""
         C.check(C.lib().mtlp_step(self._h, _stream_handle(stream), ctypes.c_float(dt)), "mtlp_step")
 
+    def integrate(self, dt, stream=None):
+        """Engine::integrate (Engine.cpp:578-595): legacy semi-implicit Euler."""
+        C.check(C.lib().mtlp_integrate(self._h, _stream_handle(stream), ctypes.c_float(dt)), "mtlp_integrate")
+
     def save_state(self, path):
         """Write positions, velocities, parameters and the sphere to `path`."""
         C.check(C.lib().mtlp_save_state(self._h, str(path).encode()), "mtlp_save_state")

@@ -107,6 +107,7 @@ def lib():
             "orc_sphere_advance": (None, [vp, PP, ctypes.c_float]),
             "orc_foam_f64": (None, [vp, vp, u32, ctypes.c_double, vp]),
             "orc_mouse_pulse_f64": (None, [vp, vp, u32, vp, vp, vp, ctypes.c_double]),
+            "orc_integrate_f64": (None, [vp, vp, u32, PP, ctypes.c_double]),
             "orc_set_threads": (None, [ctypes.c_int]),
             "orc_get_threads": (ctypes.c_int, []),
         }
@@ -428,3 +429,11 @@ def mouse_pulse(x4, v4, origin, direction, force, radius):
     lib().orc_mouse_pulse_f64(_p(x4), _p(v), len(x4), _p(d3(origin)), _p(d3(direction)), _p(d3(force)),
                               ctypes.c_double(radius))
     return v
+
+
+def integrate(x4, v4, p, dt):
+    """Legacy semi-implicit Euler step; returns new (x4, v4) in FP64."""
+    x = np.array(x4, np.float64, copy=True, order="C")
+    v = np.array(v4, np.float64, copy=True, order="C")
+    lib().orc_integrate_f64(_p(x), _p(v), len(x), ctypes.byref(p), ctypes.c_double(dt))
+    return x, v

@@ -241,6 +241,23 @@ void orc_mouse_pulse_f64(const double* x4, double* v4, uint32_t n, const double 
     }
 }
 
+void orc_integrate_f64(double* x4, double* v4, uint32_t n, const orc_params* p, double dt)
+{
+    /* Integrate.metal:11-42 with Engine.cpp:578-595's gravity and damping */
+    for (uint32_t i = 0; i < n; ++i) {
+        double* x = x4 + 4 * (size_t)i;
+        double* v = v4 + 4 * (size_t)i;
+        v[1] += -9.81 * dt;
+        for (int a = 0; a < 3; ++a) { v[a] *= 0.999; x[a] += v[a] * dt; }
+        for (int a = 0; a < 3; ++a)
+            if (x[a] < p->bounds_min[a]) { x[a] = p->bounds_min[a]; v[a] = -v[a] * 0.4; }
+        for (int a = 0; a < 3; ++a)
+            if (x[a] > p->bounds_max[a]) { x[a] = p->bounds_max[a]; v[a] = -v[a] * 0.4; }
+        x[3] = 1.0;
+        v[3] = 0.0;
+    }
+}
+
 void orc_set_threads(int threads)
 {
 #ifdef _OPENMP

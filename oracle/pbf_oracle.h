@@ -209,6 +209,9 @@ void orc_foam_f64(const double* v4, const uint32_t* nbr, uint32_t n, double max_
 void orc_mouse_pulse_f64(const double* x4, double* v4, uint32_t n, const double origin[3],
                          const double dir[3], const double force[3], double radius);
 
+/* integrate (Integrate.metal:11-42, Engine.cpp:578-595) in FP64, in place. */
+void orc_integrate_f64(double* x4, double* v4, uint32_t n, const orc_params* p, double dt);
+
 /* OpenMP thread count for the per-particle stages (0 = library default). */
 void orc_set_threads(int threads);
 int  orc_get_threads(void);

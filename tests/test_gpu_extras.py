@@ -122,3 +122,38 @@ def test_sphere_scene_kinematics_and_stability(mtlp, orc):
     lo, hi = np.array(p.bounds_min), np.array(p.bounds_max)
     assert (x[:, :3] >= lo - 1e-6).all() and (x[:, :3] <= hi + 1e-6).all()
     assert c[1] < 4.5 - 1.0  # it fell
+
+
+def test_integrate_matches_oracle_and_reference_cases(mtlp, orc):
+    """Engine::integrate: the reference's two integrator tests
+    (tests/test_integrator.cpp:47-99) and 10k random particles vs the FP64
+    oracle."""
+    with mtlp.Engine(0) as e:
+        e.reset(1, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(np.array([[0, 5, 0, 1]], np.float32), np.zeros((1, 4), np.float32))
+        x, v = np.array([[0.0, 5.0, 0.0, 1.0]]), np.zeros((1, 4))
+        op = orc.params(e.params)
+        for _ in range(100):
+            e.integrate(0.01)
+            x, v = orc.integrate(x, v, op, 0.01)
+        gx, gv = e.positions_buffer().read(), e.velocities_buffer().read()
+        assert gx[0, 1] == pytest.approx(x[0, 1], rel=1e-4) and gv[0, 1] == pytest.approx(v[0, 1], rel=1e-4)
+        e.upload_state(np.array([[0, -2.95, 0, 1]], np.float32), np.array([[0, -10, 0, 0]], np.float32))
+        e.integrate(0.05)
+        assert e.positions_buffer().read()[0, 1] == pytest.approx(-3.0) and e.velocities_buffer().read()[0, 1] > 0
+    rng = np.random.default_rng(9)
+    n = 10000
+    x = np.zeros((n, 4), np.float32)
+    x[:, :3] = rng.uniform(-3.2, 3.2, (n, 3))
+    x[:, 3] = 1
+    v = np.zeros((n, 4), np.float32)
+    v[:, :3] = rng.normal(0, 20, (n, 3))
+    with mtlp.Engine(0) as e:
+        e.reset(n, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(x, v)
+        e.integrate(1.0 / 60.0)
+        gx, gv = e.positions_buffer().read(), e.velocities_buffer().read()
+        op = orc.params(e.params)
+    rx, rv = orc.integrate(x.astype(np.float64), v.astype(np.float64), op, np.float32(1.0 / 60.0))
+    assert np.abs(gx[:, :3] - rx[:, :3]).max() < 1e-5
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() < 1e-4

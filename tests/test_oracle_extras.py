@@ -57,3 +57,21 @@ def test_mouse_pulse_formula(orc):
     assert out[1, 1] == pytest.approx(2.0 * np.exp(-0.01 / (0.04 * 0.4)))
     assert out[2, 1] == 0.0                                      # behind the origin
     assert out[3, 1] == 0.0                                      # outside the radius
+
+
+def test_integrate_reference_cases(orc):
+    """tests/test_integrator.cpp:47-99 restated: free fall for 100 steps of
+    0.01 s against the closed form of the recurrence, and a floor bounce."""
+    p = orc.params(None)
+    x = np.array([[0.0, 5.0, 0.0, 1.0]])
+    v = np.zeros((1, 4))
+    vy, py = 0.0, 5.0
+    for _ in range(100):
+        x, v = orc.integrate(x, v, p, 0.01)
+        vy = (vy - 9.81 * 0.01) * 0.999
+        py += vy * 0.01
+        if py < -3.0:
+            py, vy = -3.0, -vy * 0.4
+    assert x[0, 1] == pytest.approx(py, rel=1e-12) and v[0, 1] == pytest.approx(vy, rel=1e-12)
+    x, v = orc.integrate(np.array([[0.0, -2.95, 0.0, 1.0]]), np.array([[0.0, -10.0, 0.0, 0.0]]), p, 0.05)
+    assert x[0, 1] == -3.0 and v[0, 1] > 0.0
