@@ -153,7 +153,7 @@ mtlp_status mtlp_get_state_device(mtlp_engine* e, mtlp_stream stream, float* d_x
  * iterations x (density/lambda, delta[, sphere collision]), scatter back,
  * finalize, foam.  dt <= 0 is a no-op returning MTLP_OK. */
 mtlp_status mtlp_step(mtlp_engine* e, mtlp_stream stream, float dt);
-/* Engine::integrate(cmd, dt) (Engine.hpp:69, Engine.cpp:578-595,
+/* Engine::integrate(cmd, dt) (Engine.hpp:48, Engine.cpp:578-595,
  * Integrate.metal:11-42): the legacy semi-implicit Euler step with
  * gravity -9.81, damping 0.999 and the box with restitution 0.4; on
  * positions and velocities in place (no spatial hash). */
