@@ -46,11 +46,12 @@ __global__ __launch_bounds__(kB) void k_density_lambda_tape(float4* __restrict__
                                                            const uint32_t* __restrict__ sorted_index, StepArgs a,
                                                            float4* __restrict__ aux, float* __restrict__ lambda_orig,
                                                            uint32_t* __restrict__ nbr_orig, NbrList nl) {
+    __shared__ uint2 ent[kLdsChunks * kB];  // each thread's recorded chunks
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         density_lambda_slot<kOrig, kCount, true, true>(s, sc, cur, cell_start, sorted_index, a.n, lambda_orig,
-                                                       nbr_orig, aux, nl);
+                                                       nbr_orig, aux, nl, ent + threadIdx.x, kB);
     });
 }
 
@@ -367,6 +368,11 @@ __global__ __launch_bounds__(kBlock) void k_reduce_gravity(const double* __restr
 
 // Geometry of one pair from FP32 positions: r = p_s - p_c, 1/|r|, and the
 // spiky gradient magnitude f and its derivative f' (0 outside 1e-6 < r < h).
+// Two float4 loaded per neighbour (position and adjoint input).
+struct Pair4 {
+    float4 a, b;
+};
+
 struct PairGeo {
     float rx, ry, rz, r2, inv_r, ux, uy, uz, f, fp, f_r;
 };
@@ -475,11 +481,9 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                 atomic_sub3(prem + c, tx, ty, tz);
             }
         };
-        for_each_neighbor_of<true>(s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
-            const float4 q1 = cur[j1], q2 = cur[j2], r1 = rbar[j1], r2 = rbar[j2];
-            pair(j1, q1, r1);
-            if (two) pair(j2, q2, r2);
-        });
+        for_each_neighbor_of<true>(
+            s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], rbar[j]}; },
+            [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
         lbar[s] = lb;
         const float4 acc = pacc[s];
         pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
@@ -524,11 +528,9 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             pz += tz;
             if (!both) atomic_sub3(prem + c, tx, ty, tz);
         };
-        for_each_neighbor_of<true>(s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
-            const float4 q1 = cur[j1], q2 = cur[j2], d1 = db[j1], d2 = db[j2];
-            pair(j1, q1, d1);
-            if (two) pair(j2, q2, d2);
-        });
+        for_each_neighbor_of<true>(
+            s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], db[j]}; },
+            [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
         const float4 acc = pacc[s];
         pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
     });

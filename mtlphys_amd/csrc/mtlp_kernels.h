@@ -125,20 +125,26 @@ hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool z
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
 // Neighbour list of one solver iterate: the density pass records each
-// slot's neighbours in its stencil-walk order, and the passes that follow on
-// the same iterate (delta, and the adjoint's) walk the list instead of
+// slot's neighbours in its stencil-walk order, and the passes that follow
+// on the same iterate (delta, and the adjoint's) walk the list instead of
 // testing the 27-cell stencil's candidates again; the order is kept, so
 // their sums are bit-identical either way.  Entries go in blocks of four
-// (one 16-byte store per lane, blk[b * stride + s] holds entries 4b..4b+3
-// of slot s) so a wave's stores fill whole cache lines; count[s] may exceed
-// kMaxList, and then readers walk the stencil for that slot.
-constexpr uint32_t kMaxList = 64;
+// (blk[b * stride + s] holds entries 4b..4b+3 of slot s, the unused words
+// of the last block hold s), written by the lanes of a wave together;
+// count[s] may exceed kMaxList, and then readers walk the stencil.
+constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
     uint32_t stride;
 };
+
+// The density pass walks the stencil in chunks of at most 32 consecutive
+// candidate slots, (c0, mask) with bit k set when slot c0 + k is a
+// neighbour, and stages the first kLdsChunks of a slot's chunks in LDS (8
+// bytes each per thread) before running the terms over them.
+constexpr uint32_t kLdsChunks = 16;
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
 constexpr uint32_t kStencilBlocks[] = {128, 256, 512};

@@ -67,20 +67,54 @@ struct Rows {
     uint32_t a[9], b[9];
 };
 
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t* __restrict__ base, uint32_t i) {
+    // 32-bit byte offset from a uniform base: one VGPR per address
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + i * 4u);
+}
+
 __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* __restrict__ cell_start,
                                              const GridDev& g, uint32_t cell_off, uint32_t n) {
     Rows r;
+    uint32_t ia[9], ib[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const int z = st.z0 + k / 3, y = st.y0 + k % 3;
         const bool ok = z <= st.z1 && y <= st.y1;
         const uint32_t row = cell_off + (ok ? (uint32_t)y * g.dx + (uint32_t)z * g.dxy : 0u);
-        const uint32_t a = cell_start[row + (uint32_t)st.x0];
-        const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
-        r.a[k] = ok ? a : 0u;
-        r.b[k] = ok ? b : 0u;
+        ia[k] = row + (uint32_t)st.x0;
+        ib[k] = row + (uint32_t)st.x1 + 1u;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        r.a[k] = ld_u32(cell_start, ia[k]);
+        r.b[k] = ld_u32(cell_start, ib[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int z = st.z0 + k / 3, y = st.y0 + k % 3;
+        const bool ok = z <= st.z1 && y <= st.y1;
+        r.a[k] = ok ? r.a[k] : 0u;
+        r.b[k] = ok ? min(r.b[k], n) : 0u;
     }
     return r;
+}
+
+// Visits the stencil of slot s in walk order, one chunk of at most 32
+// consecutive candidate slots at a time: body(c0, mask), bit k of mask set
+// when slot c0 + k is a neighbour (neighbor_mask).
+template <class Body>
+__device__ __forceinline__ void for_each_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
+                                               const float4* __restrict__ cur,
+                                               const uint32_t* __restrict__ cell_start, uint32_t n,
+                                               Body&& body) {
+    const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
+    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k) {
+        const uint32_t b = rows.b[k];
+        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
+            body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+    }
 }
 
 // Visits every neighbour j of slot s (|p_s - p_j|^2 < h^2 in FP32, j != s,
@@ -91,101 +125,180 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, 
                                                   const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
                                                   Body&& body) {
+    for_each_chunk(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
+        while (mask) {
+            const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
+            mask &= mask - 1u;
+            const bool two = mask != 0u;
+            const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
+            if (two) mask &= mask - 1u;
+            body(j1, j2, two);
+        }
+    });
+}
+
+// Neighbours of slot s in walk order from chunk `first` of its stencil walk
+// on (chunks as for_each_chunk cuts them), testing one candidate at a time:
+// the rare case of more than kLdsChunks chunks, small in registers rather
+// than fast.
+template <class Fn>
+__device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
+                                                     const float4* __restrict__ cur,
+                                                     const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                     uint32_t first, Fn&& fn) {
     const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
-    const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
+    const GridDev& g = sc.g;
+    uint32_t k0 = 0;  // chunks in the rows before row k
 #pragma unroll 1
     for (int k = 0; k < 9; ++k) {
-        const uint32_t b = rows.b[k];
-        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
-            uint32_t mask = neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2);
-            while (mask) {
-                const uint32_t j1 = c0 + (uint32_t)__builtin_ctz(mask);
-                mask &= mask - 1u;
-                const bool two = mask != 0u;
-                const uint32_t j2 = two ? c0 + (uint32_t)__builtin_ctz(mask) : j1;
-                if (two) mask &= mask - 1u;
-                body(j1, j2, two);
-            }
+        const int z = st.z0 + k / 3, y = st.y0 + k % 3;
+        if (z > st.z1 || y > st.y1) continue;
+        const uint32_t row = sc.cell_off + (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
+        const uint32_t a = cell_start[row + (uint32_t)st.x0];
+        const uint32_t b = min(cell_start[row + (uint32_t)st.x1 + 1], n);
+        const uint32_t nch = b > a ? (b - a + 31u) / 32u : 0u;
+#pragma unroll 1
+        for (uint32_t c = first > k0 ? a + 32u * (first - k0) : a; c < b; ++c) {
+            const float4 q = cur[c];
+            if (c != s && dist2(pi.x - q.x, pi.y - q.y, pi.z - q.z) < sc.sd.h2) fn(c);
         }
+        k0 += nch;
     }
 }
 
-// Records a slot's neighbours in blocks of four (see NbrList): push() the
-// walk's neighbours in order, then finish().
-struct ListWriter {
-    uint4* out;       // nl.blk + s
-    uint32_t stride;
-    uint32_t cnt = 0;
-    uint32_t e0 = 0, e1 = 0, e2 = 0;
+// Walks slot s's stencil, keeping its first kLdsChunks chunks (c0, mask)
+// in this lane's LDS column le (stride lst); returns the number of chunks
+// and the number of neighbours in nbrs.
+__device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, const SceneDev& sc,
+                                                  const float4* __restrict__ cur,
+                                                  const uint32_t* __restrict__ cell_start, uint32_t n, uint2* le,
+                                                  uint32_t lst, uint32_t& nbrs) {
+    uint32_t K = 0, c = 0;
+    for_each_chunk(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
+        if (K < kLdsChunks) le[K * lst] = make_uint2(c0, mask);
+        ++K;
+        c += (uint32_t)__builtin_popcount(mask);
+    });
+    nbrs = c;
+    return K;
+}
 
-    __device__ __forceinline__ void push(uint32_t j) {
-        const uint32_t f = cnt & 3u;
-        if (f == 3u) {
-            const uint32_t b = cnt >> 2;
-            if (b < kListBlocks) out[(size_t)b * stride] = make_uint4(e0, e1, e2, j);
-        } else {
-            e0 = f == 0u ? j : e0;
-            e1 = f == 1u ? j : e1;
-            e2 = f == 2u ? j : e2;
+// Visits the set bits of the K <= kLdsChunks chunks le[0], le[lst], ... in
+// order: term(j, load(j)) per neighbour j.  Every lane moves on to its next
+// chunk by itself, so a wave runs max(neighbours) / 4 trips rather than the
+// sum over chunks of each chunk's largest count.  A trip takes four
+// neighbours, with their loads in flight before any term runs, and with
+// `listing` stores them as block t of slot s's list (padded with s), so the
+// lanes of a wave store block t together.
+template <class Load, class Term>
+__device__ __forceinline__ void visit_chunks(const uint2* le, uint32_t lst, uint32_t K, uint32_t s,
+                                             const NbrList& nl, bool listing, Load&& load, Term&& term) {
+    uint32_t k = 0, b = 0, m = 0;
+    auto next = [&](uint32_t& j) {
+        while (m == 0u && k < K) {
+            const uint2 e = le[k * lst];
+            ++k;
+            b = e.x;
+            m = e.y;
         }
-        ++cnt;
+        if (m == 0u) return false;
+        j = b + (uint32_t)__builtin_ctz(m);
+        m &= m - 1u;
+        return true;
+    };
+    uint4* out = nl.blk + s;
+    const size_t st = nl.stride;
+#pragma unroll 1
+    for (uint32_t t = 0;; ++t) {
+        uint32_t j0 = s, j1 = s, j2 = s, j3 = s;
+        if (!next(j0)) break;
+        const bool h1 = next(j1);
+        const bool h2 = h1 && next(j2);
+        const bool h3 = h2 && next(j3);
+        const auto v0 = load(j0), v1 = load(j1), v2 = load(j2), v3 = load(j3);
+        if (listing && t < kListBlocks) out[(size_t)t * st] = make_uint4(j0, j1, j2, j3);
+        term(j0, v0);
+        if (h1) term(j1, v1);
+        if (h2) term(j2, v2);
+        if (h3) term(j3, v3);
+        if (!h3) break;
     }
-    __device__ __forceinline__ void finish(uint32_t* count, uint32_t s) {
-        const uint32_t b = cnt >> 2;
-        if ((cnt & 3u) && b < kListBlocks) out[(size_t)b * stride] = make_uint4(e0, e1, e2, 0u);
-        count[s] = cnt;
+}
+
+// The cnt <= kMaxList listed neighbours of slot s in order, a block of four
+// per trip: v = load(j) for all four entries (padding holds s), then
+// term(j, v) for those below cnt.  The next block is fetched a trip ahead.
+template <class Load, class Term>
+__device__ __forceinline__ void for_each_listed(uint32_t s, uint32_t cnt, const NbrList& nl, Load&& load,
+                                                Term&& term) {
+    const uint4* p = nl.blk + s;
+    const size_t st = nl.stride;
+    uint4 e = p[0];  // read even when cnt = 0: in bounds, unused
+#pragma unroll 1
+    for (uint32_t k = 0; k < cnt; k += 4) {
+        const auto v0 = load(e.x), v1 = load(e.y), v2 = load(e.z), v3 = load(e.w);
+        const uint32_t b = min((k >> 2) + 1u, kListBlocks - 1u);
+        const uint4 ce = e;
+        e = p[(size_t)b * st];
+        term(ce.x, v0);
+        if (k + 1 < cnt) term(ce.y, v1);
+        if (k + 2 < cnt) term(ce.z, v2);
+        if (k + 3 < cnt) term(ce.w, v3);
     }
-};
+}
 
 // The neighbours of slot s in walk order: from the list when it holds them
 // all (kList and count <= kMaxList), else from the stencil.
-template <bool kList, class Body>
+template <bool kList, class Load, class Term>
 __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& pi, const SceneDev& sc,
                                                      const float4* __restrict__ cur,
                                                      const uint32_t* __restrict__ cell_start, uint32_t n,
-                                                     const NbrList& nl, Body&& body) {
+                                                     const NbrList& nl, Load&& load, Term&& term) {
     if (kList) {
         const uint32_t cnt = nl.count[s];
         if (cnt <= kMaxList) {
-            const uint4* p = nl.blk + s;
-            const size_t st = nl.stride;
-#pragma unroll 1
-            for (uint32_t k = 0; k < cnt; k += 4) {
-                const uint4 e = p[(size_t)(k >> 2) * st];
-                body(e.x, e.y, k + 1 < cnt);
-                if (k + 2 < cnt) body(e.z, e.w, k + 3 < cnt);
-            }
+            for_each_listed(s, cnt, nl, load, term);
             return;
         }
     }
-    for_each_neighbor(s, pi, sc, cur, cell_start, n, body);
+    for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
+        const auto v1 = load(j1), v2 = load(j2);
+        term(j1, v1);
+        if (two) term(j2, v2);
+    });
 }
 
 // One neighbour's contribution to the density sum (FP64) and to the
-// constraint gradient terms (Constraints.metal:110-119).
+// constraint gradient terms (Constraints.metal:110-119).  Every fused
+// multiply-add is written out (contraction off), so the result does not
+// depend on the context the compiler inlines this into.
 __device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
                                              double& wsum, float& gx, float& gy, float& gz,
                                              float& gsq) {
+#pragma clang fp contract(off)
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dist2(rx, ry, rz);
     const double dxd = rx, dyd = ry, dzd = rz;
-    const double diff = sd.h2_d - (dxd * dxd + dyd * dyd + dzd * dzd);
-    wsum += diff * diff * diff;
+    const double diff = sd.h2_d - fma(dzd, dzd, fma(dyd, dyd, dxd * dxd));
+    wsum = fma(diff, diff * diff, wsum);
     // gradient terms only enter lambda's denominator: hardware 1/sqrt (~1 ulp)
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float rl = r2 * inv_r;
-    if (rl > 1e-6f && rl < sd.h) {
-        const float dd = sd.h - rl;
-        const float sc = -sd.spiky * dd * dd * inv_r;
-        const float ax = sc * rx, ay = sc * ry, az = sc * rz;
-        gx += ax;
-        gy += ay;
-        gz += az;
-        gsq += ax * ax + ay * ay + az * az;
-    }
+    // outside (1e-6, h) the term is zero (a select: r2 = 0 makes inv_r inf)
+    const float dd = sd.h - rl;
+    const float sc = rl > 1e-6f && rl < sd.h ? -sd.spiky * dd * dd * inv_r : 0.0f;
+    const float ax = sc * rx, ay = sc * ry, az = sc * rz;
+    gx += ax;
+    gy += ay;
+    gz += az;
+    gsq = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, gsq)));
 }
 
 __device__ __forceinline__ double ipow_d(double x, uint32_t n) {
+    if (n == 4u) {  // the reference's s_corr exponent: the loop below gives (x^2)^2 too
+        const double x2 = x * x;
+        return x2 * x2;
+    }
     double result = 1.0, b = x;
     while (n) {
         if (n & 1u) result *= b;
@@ -225,14 +338,17 @@ __device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const
 // densityLambdaSorted for slot s: lambda_s into cur[s].w.  With kTape the
 // constraint-gradient sums the adjoint needs go to aux[s] = (sum_j grad W,
 // D) with D = (|sum_j grad W|^2 + sum_j |grad W|^2) / rho0^2, so that
-// lambda = -C / (D + eps).  With kList the neighbours go to nl.
+// lambda = -C / (D + eps).  With kList the neighbours go to nl; the walk's
+// chunks are staged in this lane's LDS column le (stride lst, kLdsChunks
+// entries).
 template <bool kOrig, bool kCount, bool kTape, bool kList>
 __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& sc, float4* __restrict__ cur,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ sorted_index, uint32_t n,
                                                     float* __restrict__ lambda_orig,
                                                     uint32_t* __restrict__ nbr_orig,
-                                                    float4* __restrict__ aux, const NbrList& nl) {
+                                                    float4* __restrict__ aux, const NbrList& nl,
+                                                    uint2* le = nullptr, uint32_t lst = 0) {
     const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     // sum of (h^2 - r^2)^3 in FP64; r's components are differences of
@@ -241,18 +357,26 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     double wsum = 0.0;
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
-    ListWriter lw{kList ? nl.blk + s : nullptr, nl.stride};
-    for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
-        const float4 q1 = cur[j1], q2 = cur[j2];
-        if (kList) {
-            lw.push(j1);
-            if (two) lw.push(j2);
-        }
-        cnt += two ? 2u : 1u;
-        density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
-        if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
-    });
-    if (kList) lw.finish(nl.count, s);
+    if (kList) {
+        // filter first, keeping the chunks in LDS, then the terms over them;
+        // the list for the passes that follow is written on the way.  More
+        // than kLdsChunks chunks (dense regions): the rest come from walking
+        // the stencil again and the slot gets no list.
+        const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, le, lst, cnt);
+        auto load = [&](uint32_t j) { return cur[j]; };
+        auto term = [&](uint32_t, const float4& q) { density_term(pi, q, sd, wsum, gx, gy, gz, gsq); };
+        visit_chunks(le, lst, min(K, kLdsChunks), s, nl, K <= kLdsChunks, load, term);
+        if (K > kLdsChunks)
+            neighbors_from_chunk(s, pi, sc, cur, cell_start, n, kLdsChunks, [&](uint32_t j) { term(j, load(j)); });
+        nl.count[s] = K <= kLdsChunks ? cnt : ~0u;
+    } else {
+        for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
+            cnt += two ? 2u : 1u;
+            const float4 q1 = cur[j1], q2 = cur[j2];
+            density_term(pi, q1, sd, wsum, gx, gy, gz, gsq);
+            if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
+        });
+    }
     const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
     const float dterm = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq);
     const float denom = dterm + sd.eps;
@@ -291,11 +415,9 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
     const float4 pi = cur[s];
     const double lam_d = pi.w;
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    for_each_neighbor_of<kList>(s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j1, uint32_t j2, bool two) {
-        const float4 q1 = cur[j1], q2 = cur[j2];
-        delta_term(pi, lam_d, q1, sd, dx, dy, dz);
-        if (two) delta_term(pi, lam_d, q2, sd, dx, dy, dz);
-    });
+    for_each_neighbor_of<kList>(
+        s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
+        [&](uint32_t, const float4& q) { delta_term(pi, lam_d, q, sd, dx, dy, dz); });
     dx *= sd.inv_rho0_d;
     dy *= sd.inv_rho0_d;
     dz *= sd.inv_rho0_d;

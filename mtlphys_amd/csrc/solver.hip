@@ -18,12 +18,13 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
     uint32_t* __restrict__ nbr_orig, NbrList nl) {
+    __shared__ uint2 ent[kList ? kLdsChunks * kB : 1];  // each thread's recorded chunks
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t n = a.n;
     if (s >= n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
-                                                         nbr_orig, nullptr, nl);
+                                                         nbr_orig, nullptr, nl, ent + threadIdx.x, kB);
     });
 }
 
