@@ -98,6 +98,7 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
     t.dlt = A.dlt + (size_t)slot * (I ? I : 1) * st;
     t.lblk = A.lblk + (size_t)slot * (I ? I : 1) * kListBlocks * st;
     t.lcnt = A.lcnt + (size_t)slot * (I ? I : 1) * st;
+    t.spill = e->nspill;
     t.stride = st;
     return t;
 }

@@ -166,6 +166,7 @@ struct mtlp_engine {
     // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
     uint4* nlist = nullptr;
     uint32_t* ncount = nullptr;
+    uint2* nspill = nullptr;  // kSpillChunks * n_alloc: density-pass chunk scratch
     bool use_lists = true;
     // cell buffers
     uint32_t* counts = nullptr;
@@ -200,7 +201,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list); dev_free(nlist); dev_free(ncount);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill);
         n_alloc = 0;
     }
     void free_cells() {
@@ -229,7 +230,7 @@ struct mtlp_engine {
     }
 
     NbrList nbr_list() const {
-        return use_lists ? NbrList{nlist, ncount, n_alloc} : NbrList{nullptr, nullptr, 0u};
+        return use_lists ? NbrList{nlist, ncount, n_alloc, nspill} : NbrList{nullptr, nullptr, 0u, nullptr};
     }
 
     // Kinematic sphere of the sphere-drop scene (Engine.hpp:180-184),

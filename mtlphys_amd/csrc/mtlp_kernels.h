@@ -132,19 +132,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // (blk[b * stride + s] holds entries 4b..4b+3 of slot s, the unused words
 // of the last block hold s), written by the lanes of a wave together;
 // count[s] may exceed kMaxList, and then readers walk the stencil.
+//
+// The density pass walks the stencil in chunks of at most 32 consecutive
+// candidate slots, (c0, mask) with bit k set when slot c0 + k is a
+// neighbour, keeping the first kLdsChunks of a slot's chunks in LDS (8
+// bytes each per thread) and the next kSpillChunks in spill[k * stride + s]
+// before running the terms over them; chunks beyond those (none in
+// practice) come from walking the stencil again, and the slot gets no list.
 constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
+constexpr uint32_t kLdsChunks = 16;
+constexpr uint32_t kSpillChunks = 32;
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
     uint32_t stride;
+    uint2* spill;     // kSpillChunks * stride: density-pass scratch
 };
-
-// The density pass walks the stencil in chunks of at most 32 consecutive
-// candidate slots, (c0, mask) with bit k set when slot c0 + k is a
-// neighbour, and stages the first kLdsChunks of a slot's chunks in LDS (8
-// bytes each per thread) before running the terms over them.
-constexpr uint32_t kLdsChunks = 16;
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
 constexpr uint32_t kStencilBlocks[] = {128, 256, 512};
@@ -184,12 +188,13 @@ struct StepTape {
     float4* dlt;             // I arrays
     uint4* lblk;             // I neighbour lists (kListBlocks * stride each)
     uint32_t* lcnt;          // I count arrays
+    uint2* spill;            // the density pass's chunk scratch (not part of the tape)
     size_t stride;
     float4* iterate(uint32_t k) const { return iter + k * stride; }
     float4* grad_sums(uint32_t k) const { return grad + k * stride; }
     float4* delta(uint32_t k) const { return dlt + k * stride; }
     NbrList list(uint32_t k) const {
-        return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride};
+        return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride, spill};
     }
 };
 

@@ -41,7 +41,7 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
 __device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur, uint32_t c0,
                                                   uint32_t c1, uint32_t s, float4 pi, float h2) {
     const float4* q = cur + c0;
-    const uint32_t len = c1 - c0;
+    const uint32_t len = c1 - c0;  // >= 1
     uint32_t mask = 0;
 #pragma unroll
     for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
@@ -51,10 +51,13 @@ __device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur
         for (uint32_t u = 0; u < 8; ++u) v[u] = q[k0 + u];
 #pragma unroll
         for (uint32_t u = 0; u < 8; ++u) {
-            const float r2 = dist2(pi.x - v[u].x, pi.y - v[u].y, pi.z - v[u].z);
-            mask |= r2 < h2 ? (1u << (k0 + u)) : 0u;
+            // shift in the sign of r2 - h2 (exact in FP32): set when r2 < h2
+            const float d = dist2(pi.x - v[u].x, pi.y - v[u].y, pi.z - v[u].z) - h2;
+            mask = __builtin_amdgcn_alignbit(mask, __float_as_uint(d), 31);
         }
     }
+    // candidate k went in at bit 8 * trips - 1 - k
+    mask = __builtin_bitreverse32(mask) >> (32u - 8u * ((min(len, 32u) + 7u) / 8u));
     mask &= len >= 32u ? 0xffffffffu : (1u << len) - 1u;
     const uint32_t self = s - c0;  // wraps to a large value when s < c0
     return self < 32u ? mask & ~(1u << self) : mask;
@@ -139,8 +142,8 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, 
 
 // Neighbours of slot s in walk order from chunk `first` of its stencil walk
 // on (chunks as for_each_chunk cuts them), testing one candidate at a time:
-// the rare case of more than kLdsChunks chunks, small in registers rather
-// than fast.
+// the case of more chunks than the density pass keeps, small in registers
+// rather than fast.
 template <class Fn>
 __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
                                                      const float4* __restrict__ cur,
@@ -167,15 +170,18 @@ __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& p
 }
 
 // Walks slot s's stencil, keeping its first kLdsChunks chunks (c0, mask)
-// in this lane's LDS column le (stride lst); returns the number of chunks
-// and the number of neighbours in nbrs.
+// in this lane's LDS column le (stride lst) and the next kSpillChunks in
+// nl.spill; returns the number of chunks and the number of neighbours in
+// nbrs.
 __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
-                                                  const uint32_t* __restrict__ cell_start, uint32_t n, uint2* le,
-                                                  uint32_t lst, uint32_t& nbrs) {
+                                                  const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                  const NbrList& nl, uint2* le, uint32_t lst, uint32_t& nbrs) {
     uint32_t K = 0, c = 0;
     for_each_chunk(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
-        if (K < kLdsChunks) le[K * lst] = make_uint2(c0, mask);
+        const uint2 e = make_uint2(c0, mask);
+        if (K < kLdsChunks) le[K * lst] = e;
+        else if (K < kLdsChunks + kSpillChunks) nl.spill[(size_t)(K - kLdsChunks) * nl.stride + s] = e;
         ++K;
         c += (uint32_t)__builtin_popcount(mask);
     });
@@ -183,8 +189,9 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
     return K;
 }
 
-// Visits the set bits of the K <= kLdsChunks chunks le[0], le[lst], ... in
-// order: term(j, load(j)) per neighbour j.  Every lane moves on to its next
+// Visits the set bits of the K <= kLdsChunks + kSpillChunks recorded chunks
+// (le[0], le[lst], ..., then nl.spill) in order: term(j, load(j)) per
+// neighbour j.  Every lane moves on to its next
 // chunk by itself, so a wave runs max(neighbours) / 4 trips rather than the
 // sum over chunks of each chunk's largest count.  A trip takes four
 // neighbours, with their loads in flight before any term runs, and with
@@ -196,7 +203,9 @@ __device__ __forceinline__ void visit_chunks(const uint2* le, uint32_t lst, uint
     uint32_t k = 0, b = 0, m = 0;
     auto next = [&](uint32_t& j) {
         while (m == 0u && k < K) {
-            const uint2 e = le[k * lst];
+            uint2 e;
+            if (k < kLdsChunks) e = le[k * lst];
+            else e = nl.spill[(size_t)(k - kLdsChunks) * nl.stride + s];
             ++k;
             b = e.x;
             m = e.y;
@@ -358,17 +367,16 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     float gx = 0.f, gy = 0.f, gz = 0.f, gsq = 0.f;
     uint32_t cnt = 0;
     if (kList) {
-        // filter first, keeping the chunks in LDS, then the terms over them;
-        // the list for the passes that follow is written on the way.  More
-        // than kLdsChunks chunks (dense regions): the rest come from walking
-        // the stencil again and the slot gets no list.
-        const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, le, lst, cnt);
+        // filter first, keeping the chunks, then the terms over them; the
+        // list for the passes that follow is written on the way
+        constexpr uint32_t kKept = kLdsChunks + kSpillChunks;
+        const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, nl, le, lst, cnt);
         auto load = [&](uint32_t j) { return cur[j]; };
         auto term = [&](uint32_t, const float4& q) { density_term(pi, q, sd, wsum, gx, gy, gz, gsq); };
-        visit_chunks(le, lst, min(K, kLdsChunks), s, nl, K <= kLdsChunks, load, term);
-        if (K > kLdsChunks)
-            neighbors_from_chunk(s, pi, sc, cur, cell_start, n, kLdsChunks, [&](uint32_t j) { term(j, load(j)); });
-        nl.count[s] = K <= kLdsChunks ? cnt : ~0u;
+        visit_chunks(le, lst, min(K, kKept), s, nl, K <= kKept, load, term);
+        if (K > kKept)
+            neighbors_from_chunk(s, pi, sc, cur, cell_start, n, kKept, [&](uint32_t j) { term(j, load(j)); });
+        nl.count[s] = K <= kKept ? cnt : ~0u;
     } else {
         for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
             cnt += two ? 2u : 1u;
