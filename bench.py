@@ -231,6 +231,9 @@ def main():
     eng.upload_state(x, v)
     cells = eng.total_cells
     stream = torch.cuda.current_stream()
+    # workgroup size of the stencil kernels for this workload (the engine's
+    # autotuner: one timed step per candidate size, state restored after)
+    block = eng.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"), stream)
 
     for _ in range(args.warmup):
         eng.step(DT, stream)
@@ -301,7 +304,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic: PCG32-jittered lattice dam break, seed 1",
         "config": {"workload": "dam_break_1m", "particles": N_PARTICLES, "h": params.h, "cells": cells,
-                   "grid": list(mtlp.grid_dims(params)), "iterations": iters, "dt": DT,
+                   "grid": list(mtlp.grid_dims(params)), "iterations": iters, "dt": DT, "stencil_block": block,
                    "alg_bytes_per_step": b_step},
         "roofline": roofline,
         "kernels": kernels,

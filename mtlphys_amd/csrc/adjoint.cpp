@@ -117,6 +117,7 @@ mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t
     b.cell_start = const_cast<uint32_t*>(t.cell_start);
     b.sorted_index = const_cast<uint32_t*>(t.sorted_index);
     b.sorted = t.iterate(0);
+    b.slot_of = nullptr;
     StageClock clk(e, s);
     MTLP_TRY(enqueue_hash(e, s, a, clk, b));
     const uint32_t I = A.tape_iters;

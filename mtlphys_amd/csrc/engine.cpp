@@ -208,6 +208,7 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->foam, na));
     HIP_TRY(dev_alloc(&e->tmp_index, na));
     HIP_TRY(dev_alloc(&e->sorted_index, na));
+    HIP_TRY(dev_alloc(&e->slot_of, na));
     HIP_TRY(dev_alloc(&e->sorted[0], na + kChunkSlack));  // the stencil filter reads past the end
     HIP_TRY(dev_alloc(&e->sorted[1], na + kChunkSlack));
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
@@ -311,10 +312,8 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     const uint32_t iters = e->iterations();
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
-        const bool last = it + 1 == iters;
-        HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a,
-                                      last ? e->lambdas : nullptr, it == 0 ? e->nbr : nullptr,
-                                      e->nbr_list(), e->stencil_block, s));
+        HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a, nullptr,
+                                      it == 0 ? e->nbr : nullptr, e->nbr_list(), e->stencil_block, s));
         clk.mark("density_lambda");
         HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], e->nbr_list(),
                                    e->stencil_block, s));
@@ -325,7 +324,9 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
         HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
         clk.mark("count_neighbors");
     }
-    HIP_TRY(launch_finalize(e->sorted[cur], e->sorted_index, a, e->x, e->v, e->prev, e->nbr, e->foam, s));
+    // lambda of the last density pass is in .w of the other iterate buffer
+    HIP_TRY(launch_finalize(e->sorted[cur], e->slot_of, a, e->x, e->v, e->prev, e->nbr, e->foam,
+                            iters ? e->sorted[cur ^ 1] : nullptr, e->lambdas, s));
     clk.mark("scatter_finalize");
     e->final_buf = cur;
     return MTLP_OK;
@@ -732,14 +733,18 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
     for (uint32_t b : kStencilBlocks) {
         e->stencil_block = b;
         float ms = 0.0f;
-        for (int rep = 0; rep < 2 && st == MTLP_OK; ++rep) {  // rep 0 warms up
+        // rep 0 warms up with one step; rep 1 times kTuneSteps steps, all
+        // from the same saved state (a single step from a fresh state can
+        // favour a size that later steps do not)
+        constexpr int kTuneSteps = 4;
+        for (int rep = 0; rep < 2 && st == MTLP_OK; ++rep) {
             if (hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
                 hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
                 hipEventRecord(t0, s) != hipSuccess) {
                 st = fail(MTLP_E_HIP, "autotune copy/event failed");
                 break;
             }
-            st = enqueue_step(e, s, dt);
+            for (int k = 0; k < (rep ? kTuneSteps : 1) && st == MTLP_OK; ++k) st = enqueue_step(e, s, dt);
             if (st != MTLP_OK || hipEventRecord(t1, s) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
                 hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
                 if (st == MTLP_OK) st = fail(MTLP_E_HIP, "autotune timing failed");

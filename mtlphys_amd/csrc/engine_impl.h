@@ -160,6 +160,7 @@ struct mtlp_engine {
     // particle buffers (sorted order)
     uint32_t* tmp_index = nullptr;
     uint32_t* sorted_index = nullptr;
+    uint32_t* slot_of = nullptr;  // sorted slot of each particle
     float4* sorted[2] = {nullptr, nullptr};
     uint32_t* big_list = nullptr;
     int final_buf = 0;
@@ -201,7 +202,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill);
         n_alloc = 0;
     }
     void free_cells() {
@@ -226,6 +227,7 @@ struct mtlp_engine {
         b.x = x; b.v = v; b.keys = keys; b.tickets = tickets; b.counts = counts;
         b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
+        b.slot_of = slot_of;
         return b;
     }
 

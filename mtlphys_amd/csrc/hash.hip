@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted, uint32_t* __restrict__ big_list, uint32_t* __restrict__ big_count,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = a.n;
     if (s >= n) return;
@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const uint32_t dst = lo + rank;
     sorted_index[dst] = i;
     sorted[dst] = binned_pos(x, v, i, a);
+    if (slot_of) slot_of[i] = dst;
 }
 
 // One workgroup per listed cell (grid-stride over the list).
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, const uint32_t* __restrict__ big_list,
     const uint32_t* __restrict__ big_count, uint32_t* __restrict__ sorted_index,
-    float4* __restrict__ sorted) {
+    float4* __restrict__ sorted, uint32_t* __restrict__ slot_of) {
     __shared__ uint32_t lds[kLdsSortMax];
     __shared__ uint32_t s_wave[kBigBlock / 64];
     const uint32_t t = threadIdx.x;
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
                 const uint32_t i = lds[k];
                 sorted_index[lo + k] = i;
                 sorted[lo + k] = binned_pos(x, v, i, a);
+                if (slot_of) slot_of[i] = lo + k;
             }
             __syncthreads();
         } else {
@@ -153,6 +155,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
                     if (dst < hi) {
                         sorted_index[dst] = j;
                         sorted[dst] = binned_pos(x, v, j, a);
+                        if (slot_of) slot_of[j] = dst;
                     }
                 }
                 run += tot;
@@ -224,13 +227,13 @@ hipError_t launch_scatter_gather(const HashBufs& b, const StepArgs& a, hipStream
                        b.cell_start, a.n, b.tmp_index, b.err);
     hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp_index, b.keys,
                        b.cell_start, b.x, b.v, a, b.sorted_index, b.sorted, b.big_list,
-                       b.big_count, b.err);
+                       b.big_count, b.err, b.slot_of);
     // Only cells above kSmallCell particles exist past this point; with
     // n <= kSmallCell there can be none.
     if (a.n > kSmallCell)
         hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp_index, b.keys,
                            b.cell_start, b.x, b.v, a, b.big_list, b.big_count, b.sorted_index,
-                           b.sorted);
+                           b.sorted, b.slot_of);
     return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ struct HashBufs {
     uint32_t* big_list;   // cells larger than kSmallCell
     uint32_t* big_count;
     uint32_t* err;
+    uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------
@@ -142,7 +143,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
 constexpr uint32_t kLdsChunks = 16;
-constexpr uint32_t kSpillChunks = 32;
+constexpr uint32_t kSpillChunks = 16;
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
@@ -164,9 +165,13 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
 // Scatter back + finalize; with foam non-NULL also computeFoam from the
 // step's neighbour counts (nbr, original order).
-hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
+// Scatter back + finalize in original order: particle i takes its result
+// from slot slot_of[i]; with lam_sorted (the last density iterate, lambda in
+// .w) it also writes lambda_orig.
+hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
-                           const uint32_t* nbr, float* foam, hipStream_t s);
+                           const uint32_t* nbr, float* foam, const float4* lam_sorted, float* lambda_orig,
+                           hipStream_t s);
 // integrate (Integrate.metal:11-42) on n particles; bounds from b.
 hipError_t launch_integrate(float4* x, float4* v, uint32_t n, float dt, float gy, float damping,
                             const FinalDev& b, hipStream_t s);

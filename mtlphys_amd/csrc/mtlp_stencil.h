@@ -445,22 +445,19 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
     next[s] = q;
 }
 
-// Scatter back + finalize for slot s, particle i = sorted_index[s]: x_in[i]
-// is the position at the start of the step; the step's result goes to
-// x_out[i], v_out[i] (x_out may be x_in), and x_in[i] to prev[i] if given.
-// With foam, also the foam intensity from nbr[i] and the new velocity.
-__device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, float dt,
-                                              const float4* __restrict__ fin,
-                                              const uint32_t* __restrict__ sorted_index,
-                                              const float4* x_in, float4* x_out, float4* __restrict__ v_out,
-                                              float4* __restrict__ prev, const uint32_t* __restrict__ nbr = nullptr,
-                                              float* __restrict__ foam = nullptr) {
+// finalizeStep (Constraints.metal:305-353) for one particle: pf is its
+// final iterate, old its position at the start of the step; returns the new
+// position and velocity, and with want_foam the foam intensity from its
+// neighbour count (computeFoam, Constraints.metal:287-299).
+struct Finalized {
+    float4 x, v;
+    float foam;
+};
+
+__device__ __forceinline__ Finalized finalize_particle(const FinalDev& fd, float dt, const float4& pf,
+                                                       const float4& old, bool want_foam, uint32_t nbr) {
     const float inv_dt = 1.0f / dt;       // Engine.cpp:561
     const float vmax = fd.vmax_h * inv_dt;  // Constraints.metal:347
-    const uint32_t i = sorted_index[s];
-    const float4 pf = fin[s];
-    const float4 old = x_in[i];
-    if (prev) prev[i] = old;
     const float px = fminf(fmaxf(pf.x, fd.bminx), fd.bmaxx);
     const float py = fminf(fmaxf(pf.y, fd.bminy), fd.bmaxy);
     const float pz = fminf(fmaxf(pf.z, fd.bminz), fd.bmaxz);
@@ -491,16 +488,34 @@ __device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, fl
         vy *= f;
         vz *= f;
     }
-    x_out[i] = make_float4(px, py, pz, 1.0f);
-    v_out[i] = make_float4(vx, vy, vz, 0.0f);
-    if (foam) {
-        // computeFoam (Constraints.metal:287-299): surface-ness from the
-        // neighbour count times smoothstep(1.2, 4.5, |v|)
+    Finalized r{make_float4(px, py, pz, 1.0f), make_float4(vx, vy, vz, 0.0f), 0.0f};
+    if (want_foam) {
+        // surface-ness from the neighbour count times smoothstep(1.2, 4.5, |v|)
         const float speed = sqrtf(vx * vx + vy * vy + vz * vz);
-        const float surface = 1.0f - fminf(fmaxf((float)nbr[i] / fd.foam_max_nbr, 0.0f), 1.0f);
+        const float surface = 1.0f - fminf(fmaxf((float)nbr / fd.foam_max_nbr, 0.0f), 1.0f);
         const float t = fminf(fmaxf((speed - 1.2f) / (4.5f - 1.2f), 0.0f), 1.0f);
-        foam[i] = surface * (t * t * (3.0f - 2.0f * t));
+        r.foam = surface * (t * t * (3.0f - 2.0f * t));
     }
+    return r;
+}
+
+// Scatter back + finalize for slot s, particle i = sorted_index[s]: x_in[i]
+// is the position at the start of the step; the step's result goes to
+// x_out[i], v_out[i] (x_out may be x_in), and x_in[i] to prev[i] if given.
+// With foam, also the foam intensity from nbr[i] and the new velocity.
+__device__ __forceinline__ void finalize_slot(uint32_t s, const FinalDev& fd, float dt,
+                                              const float4* __restrict__ fin,
+                                              const uint32_t* __restrict__ sorted_index,
+                                              const float4* x_in, float4* x_out, float4* __restrict__ v_out,
+                                              float4* __restrict__ prev, const uint32_t* __restrict__ nbr = nullptr,
+                                              float* __restrict__ foam = nullptr) {
+    const uint32_t i = sorted_index[s];
+    const float4 old = x_in[i];
+    if (prev) prev[i] = old;
+    const Finalized r = finalize_particle(fd, dt, fin[s], old, foam != nullptr, foam ? nbr[i] : 0u);
+    x_out[i] = r.x;
+    v_out[i] = r.v;
+    if (foam) foam[i] = r.foam;
 }
 
 inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }

@@ -40,17 +40,25 @@ __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ c
     });
 }
 
-// Scatter back + finalize (+ computeFoam): slot s writes particle
-// sorted_index[s].  The old position (still in x) becomes prev; x and v get
-// the step's result.
+// Scatter back + finalize (+ computeFoam), in original order: particle i
+// reads its final iterate at slot slot_of[i]; its old position (still in x)
+// becomes prev; x and v get the step's result.  With lam_sorted, lambda of
+// the last density pass (in .w of that iterate) goes to lambda_orig[i].
 __global__ __launch_bounds__(kBlock) void k_finalize(
-    const float4* __restrict__ fin, const uint32_t* __restrict__ sorted_index, StepArgs a,
-    float4* x, float4* __restrict__ v, float4* __restrict__ prev, const uint32_t* __restrict__ nbr,
-    float* __restrict__ foam) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= a.n) return;
-    with_scene(s, a, [&](const SceneDev& sc) {
-        finalize_slot(s, sc.fd, a.dt, fin, sorted_index, x, x, v, prev, nbr, foam);
+    const float4* __restrict__ fin, const uint32_t* __restrict__ slot_of, StepArgs a,
+    float4* __restrict__ x, float4* __restrict__ v, float4* __restrict__ prev, const uint32_t* __restrict__ nbr,
+    float* __restrict__ foam, const float4* __restrict__ lam_sorted, float* __restrict__ lambda_orig) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    with_scene(i, a, [&](const SceneDev& sc) {
+        const uint32_t s = slot_of[i];
+        const float4 old = x[i];
+        prev[i] = old;
+        const Finalized r = finalize_particle(sc.fd, a.dt, fin[s], old, foam != nullptr, foam ? nbr[i] : 0u);
+        x[i] = r.x;
+        v[i] = r.v;
+        if (foam) foam[i] = r.foam;
+        if (lam_sorted) lambda_orig[i] = lam_sorted[s].w;
     });
 }
 
@@ -165,12 +173,13 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const float4* final_sorted, const uint32_t* sorted_index,
+hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
-                           const uint32_t* nbr, float* foam, hipStream_t s) {
+                           const uint32_t* nbr, float* foam, const float4* lam_sorted, float* lambda_orig,
+                           hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, final_sorted,
-                       sorted_index, a, x, v, prev, nbr, foam);
+                       slot_of, a, x, v, prev, nbr, foam, lam_sorted, lambda_orig);
     return hipGetLastError();
 }
 
