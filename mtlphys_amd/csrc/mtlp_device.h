@@ -128,6 +128,28 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// Bitonic sort of the wave's 64 (key, val) pairs by key, then val; lane l
+// returns the l-th val.  Every lane of the wave must take part.
+__device__ __forceinline__ uint32_t wave_sort_by_key(uint32_t key, uint32_t val) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t ok = (uint32_t)__shfl_xor((int)key, (int)j);
+            const uint32_t ov = (uint32_t)__shfl_xor((int)val, (int)j);
+            const bool ascending = (lane & k) == 0u, lower = (lane & j) == 0u;
+            const bool other_less = ok < key || (ok == key && ov < val);
+            // the lower lane of the pair keeps the smaller pair when ascending
+            if (lower ? other_less == ascending : other_less != ascending) {
+                key = ok;
+                val = ov;
+            }
+        }
+    }
+    return val;
+}
+
 // MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own L2.
 // Renumber so that XCD x runs one contiguous run of logical blocks: slots
 // that share stencil rows then share an L2.  A bijection on [0, nb).

@@ -19,8 +19,22 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
     uint32_t* __restrict__ nbr_orig, NbrList nl) {
     __shared__ uint2 ent[kList ? kLdsChunks * kB : 1];  // each thread's recorded chunks
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t n = a.n;
+    // The wave's slots are handed to its lanes in order of their current
+    // cell, so lanes that walk the same stencil rows sit together: their
+    // candidate loads then touch fewer cache lines per instruction.  Only
+    // the lane assignment changes.  (Full waves inside one scene.)
+    uint32_t s = s0;
+    const uint32_t wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)(s0 - __lane_id()));
+    if (wbase + 64u <= n) {
+        const uint32_t sid = scene_of(wbase, a);
+        if (sid == scene_of(wbase + 63u, a)) {
+            const SceneDev sc = load_scene_uniform(a.scenes, sid);
+            const float4 p = cur[s0];
+            s = wave_sort_by_key(key_of(cell_of(p.x, p.y, p.z, sc.g), sc.g), s0);
+        }
+    }
     if (s >= n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
