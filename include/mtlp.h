@@ -236,7 +236,7 @@ mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n
  * graph and replays it (the look-back words of the scan are cleared by a
  * kernel inside the graph).  Profiling is ignored while graphs are on. */
 mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
-/* Times four steps at each stencil workgroup size from the current state
+/* Times eight steps at each stencil workgroup size from the current state
  * (restoring it afterwards) and keeps the fastest; with cache_path, reads a
  * previous choice for the same batch shape from that text file or appends
  * the new one.  *block receives the chosen size (may be NULL). */

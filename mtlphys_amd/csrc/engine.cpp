@@ -733,10 +733,10 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
     for (uint32_t b : kStencilBlocks) {
         e->stencil_block = b;
         float ms = 0.0f;
-        // rep 0 warms up with one step; rep 1 times kTuneSteps steps, all
+        // rep 0 warms up with one step; rep 1 times kTuneSteps steps, both
         // from the same saved state (a single step from a fresh state can
         // favour a size that later steps do not)
-        constexpr int kTuneSteps = 4;
+        constexpr int kTuneSteps = 8;
         for (int rep = 0; rep < 2 && st == MTLP_OK; ++rep) {
             if (hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
                 hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||

@@ -91,8 +91,8 @@ def make_ensemble(n_scenes=256, seed=3, device=0):
 
 
 def step_bytes(n, cells, iters):
-    """Compulsory bytes of one step: (100 + 44 I) N + (16 + 8 I) C."""
-    return (100 + 44 * iters) * n + (16 + 8 * iters) * cells
+    """Compulsory bytes of one step: (152 + 44 I) N + (16 + 8 I) C."""
+    return (152 + 44 * iters) * n + (16 + 8 * iters) * cells
 
 
 __all__ = ["DT", "ADJ_STEPS", "ADJ_CKPT", "ADJ_SHIFT", "differentiable", "dam_break", "cube_drop", "ensemble_config", "make_ensemble", "step_bytes",
