@@ -305,6 +305,65 @@ def test_step_kernel_level_after_settling(mtlp, orc):
     _kernel_level(mtlp, orc, p, x1, v1, 1.0 / 120.0)
 
 
+def _chunks_and_counts(orc, params, x, v, dt):
+    """Per particle at iteration 0: stencil chunks (rows of <= 32 candidate
+    slots, as the density pass cuts them) and neighbour count."""
+    op = orc.params(params)
+    g = orc.grid(op)
+    xs = orc.predict(x, v, op, dt)
+    bh = orc.build_spatial_hash(xs, g, np.float32(op.h) * np.float32(op.h))
+    cs = bh["cell_start"].astype(np.int64)
+    sx = bh["sorted4"][:, :3]
+    dims = np.array(g.dim, np.int64)
+    c = np.clip(np.floor((sx - np.array(op.bounds_min, np.float32)) / np.float32(op.cell_size)).astype(np.int64),
+                0, dims - 1)
+    K = np.zeros(len(x), np.int64)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            z, y = c[:, 2] + dz, c[:, 1] + dy
+            ok = (z >= 0) & (z < dims[2]) & (y >= 0) & (y < dims[1])
+            x0, x1 = np.maximum(c[:, 0] - 1, 0), np.minimum(c[:, 0] + 1, dims[0] - 1)
+            row = dims[0] * (np.clip(y, 0, dims[1] - 1) + dims[1] * np.clip(z, 0, dims[2] - 1))
+            K += np.where(ok, (cs[row + x1 + 1] - cs[row + x0] + 31) // 32, 0)
+    return K, bh["nbr"]
+
+
+def test_dense_regions_chunk_paths(mtlp, orc):
+    """Blobs of three densities drive the density pass through every way it
+    keeps a slot's stencil chunks: all in LDS (<= 16), the global scratch
+    (17..32, with and without a neighbour list of <= 96 entries), and the
+    second stencil walk (> 32).  Lists on and off must agree bit for bit,
+    and each iteration's kernels must match the FP64 oracle."""
+    n = 20000
+    p = mtlp.scene_params(3, n)
+    x, v = mtlp.generate_scene(3, n, 5, p)
+    rng = np.random.default_rng(8)
+    centre = x[:, :3].mean(axis=0)
+    h = p.h
+    for lo, hi, side, off in ((0, 600, 1.2, (-2.5, 0.0, 0.0)), (600, 3600, 4.0, (2.5, 0.0, 0.0)),
+                              (3600, 7000, 6.0, (0.0, 0.0, 3.0))):
+        x[lo:hi, :3] = (centre + np.array(off) * h + rng.uniform(-0.5, 0.5, (hi - lo, 3)) * side * h).astype(
+            np.float32)
+    dt = 1.0 / 120.0
+    K, cnt = _chunks_and_counts(orc, p, x, v, dt)
+    assert (K <= 16).sum() > 1000
+    assert ((K > 16) & (K <= 32) & (cnt <= 96)).sum() > 50
+    assert ((K > 16) & (K <= 32) & (cnt > 96)).sum() > 50
+    assert (K > 32).sum() > 50
+    outs = {}
+    for lists in (False, True):
+        with mtlp.Engine(0, p) as e:
+            e.reset(n, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+            e.set_neighbor_lists(lists)
+            e.step(dt)
+            outs[lists] = (e.positions_buffer().read(), e.velocities_buffer().read(), e.lambdas_buffer().read())
+            assert e.error_flags() == 0
+    for a, b in zip(outs[False], outs[True]):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    _kernel_level(mtlp, orc, p, x, v, dt)
+
+
 def test_step_free_running_vs_fp64(mtlp, orc):
     """One full step from the same state: the GPU may differ from the FP64
     oracle by no more than a few times the oracle's own FP32 instantiation
