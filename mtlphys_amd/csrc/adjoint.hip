@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                 atomic_sub3(prem + c, tx, ty, tz);
             }
         };
-        for_each_neighbor_of<true>(
+        for_each_neighbor_of<true, false>(
             s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], rbar[j]}; },
             [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
         lbar[s] = lb;
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             pz += tz;
             if (!both) atomic_sub3(prem + c, tx, ty, tz);
         };
-        for_each_neighbor_of<true>(
+        for_each_neighbor_of<true, false>(
             s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], db[j]}; },
             [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
         const float4 acc = pacc[s];

@@ -214,7 +214,7 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
     HIP_TRY(dev_alloc(&e->nlist, (size_t)kListBlocks * na));
     HIP_TRY(dev_alloc(&e->ncount, na));
-    HIP_TRY(dev_alloc(&e->nspill, (size_t)kSpillChunks * na));
+    HIP_TRY(dev_alloc(&e->nspill, (size_t)kChunkScratch * na));
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));

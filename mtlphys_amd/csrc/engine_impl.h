@@ -167,7 +167,7 @@ struct mtlp_engine {
     // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
     uint4* nlist = nullptr;
     uint32_t* ncount = nullptr;
-    uint2* nspill = nullptr;  // kSpillChunks * n_alloc: density-pass chunk scratch
+    uint2* nspill = nullptr;  // kChunkScratch * n_alloc: density-pass chunk scratch
     bool use_lists = true;
     // cell buffers
     uint32_t* counts = nullptr;

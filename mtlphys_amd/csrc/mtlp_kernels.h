@@ -137,18 +137,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // The density pass walks the stencil in chunks of at most 32 consecutive
 // candidate slots, (c0, mask) with bit k set when slot c0 + k is a
 // neighbour, keeping the first kLdsChunks of a slot's chunks in LDS (8
-// bytes each per thread) and the next kSpillChunks in spill[k * stride + s]
-// before running the terms over them; chunks beyond those (none in
-// practice) come from walking the stencil again, and the slot gets no list.
+// bytes each per thread) and chunks kLdsChunks .. kChunkScratch - 1 in
+// spill[k * stride + s], before running the terms over them.  A slot with
+// more than kMaxList neighbours gets count kCountChunks | K and all its K
+// <= kChunkScratch chunks in spill, which the delta pass right after walks
+// (the scratch is rewritten by the next density pass); more chunks than
+// that (none in practice) gives kCountStencil, and readers walk the stencil.
 constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
 constexpr uint32_t kLdsChunks = 16;
-constexpr uint32_t kSpillChunks = 16;
+constexpr uint32_t kChunkScratch = 32;
+constexpr uint32_t kCountChunks = 0x80000000u;
+constexpr uint32_t kCountStencil = 0xffffffffu;
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
     uint32_t stride;
-    uint2* spill;     // kSpillChunks * stride: density-pass scratch
+    uint2* spill;     // kChunkScratch * stride: density-pass chunk scratch
 };
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).

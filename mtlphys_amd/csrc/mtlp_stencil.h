@@ -142,8 +142,8 @@ __device__ __forceinline__ void for_each_neighbor(uint32_t s, const float4& pi, 
 
 // Neighbours of slot s in walk order from chunk `first` of its stencil walk
 // on (chunks as for_each_chunk cuts them), testing one candidate at a time:
-// the case of more chunks than the density pass keeps, small in registers
-// rather than fast.
+// the case of more chunks than the density pass keeps (kChunkScratch), small
+// in registers rather than fast.
 template <class Fn>
 __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
                                                      const float4* __restrict__ cur,
@@ -170,9 +170,9 @@ __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& p
 }
 
 // Walks slot s's stencil, keeping its first kLdsChunks chunks (c0, mask)
-// in this lane's LDS column le (stride lst) and the next kSpillChunks in
-// nl.spill; returns the number of chunks and the number of neighbours in
-// nbrs.
+// in this lane's LDS column le (stride lst) and chunks kLdsChunks ..
+// kChunkScratch - 1 at their index in nl.spill; returns the number of
+// chunks and the number of neighbours in nbrs.
 __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
@@ -181,7 +181,7 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
     for_each_chunk(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
         const uint2 e = make_uint2(c0, mask);
         if (K < kLdsChunks) le[K * lst] = e;
-        else if (K < kLdsChunks + kSpillChunks) nl.spill[(size_t)(K - kLdsChunks) * nl.stride + s] = e;
+        else if (K < kChunkScratch) nl.spill[(size_t)K * nl.stride + s] = e;
         ++K;
         c += (uint32_t)__builtin_popcount(mask);
     });
@@ -189,23 +189,20 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
     return K;
 }
 
-// Visits the set bits of the K <= kLdsChunks + kSpillChunks recorded chunks
-// (le[0], le[lst], ..., then nl.spill) in order: term(j, load(j)) per
-// neighbour j.  Every lane moves on to its next
-// chunk by itself, so a wave runs max(neighbours) / 4 trips rather than the
-// sum over chunks of each chunk's largest count.  A trip takes four
-// neighbours, with their loads in flight before any term runs, and with
-// `listing` stores them as block t of slot s's list (padded with s), so the
-// lanes of a wave store block t together.
-template <class Load, class Term>
-__device__ __forceinline__ void visit_chunks(const uint2* le, uint32_t lst, uint32_t K, uint32_t s,
-                                             const NbrList& nl, bool listing, Load&& load, Term&& term) {
+// Visits the set bits of K <= kChunkScratch chunks (c0, mask) in order,
+// chunk k from chunk(k): term(j, load(j)) per neighbour j.  Every lane moves
+// on to its next chunk by itself, so a wave runs max(neighbours) / 4 trips
+// rather than the sum over chunks of each chunk's largest count.  A trip
+// takes four neighbours, with their loads in flight before any term runs,
+// and with `listing` stores them as block t of slot s's list (padded with
+// s), so the lanes of a wave store block t together.
+template <class Chunk, class Load, class Term>
+__device__ __forceinline__ void visit_chunks(Chunk&& chunk, uint32_t K, uint32_t s, const NbrList& nl,
+                                             bool listing, Load&& load, Term&& term) {
     uint32_t k = 0, b = 0, m = 0;
     auto next = [&](uint32_t& j) {
         while (m == 0u && k < K) {
-            uint2 e;
-            if (k < kLdsChunks) e = le[k * lst];
-            else e = nl.spill[(size_t)(k - kLdsChunks) * nl.stride + s];
+            const uint2 e = chunk(k);
             ++k;
             b = e.x;
             m = e.y;
@@ -257,8 +254,10 @@ __device__ __forceinline__ void for_each_listed(uint32_t s, uint32_t cnt, const 
 }
 
 // The neighbours of slot s in walk order: from the list when it holds them
-// all (kList and count <= kMaxList), else from the stencil.
-template <bool kList, class Load, class Term>
+// all (kList and count <= kMaxList); with kScratch (the pass right after the
+// density pass that wrote the list) from the chunks the density pass left in
+// nl.spill when the count says so; else from the stencil.
+template <bool kList, bool kScratch, class Load, class Term>
 __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& pi, const SceneDev& sc,
                                                      const float4* __restrict__ cur,
                                                      const uint32_t* __restrict__ cell_start, uint32_t n,
@@ -267,6 +266,11 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
         const uint32_t cnt = nl.count[s];
         if (cnt <= kMaxList) {
             for_each_listed(s, cnt, nl, load, term);
+            return;
+        }
+        if (kScratch && (cnt & kCountChunks) && cnt != kCountStencil) {
+            visit_chunks([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, cnt & ~kCountChunks, s,
+                         nl, false, load, term);
             return;
         }
     }
@@ -369,14 +373,29 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     if (kList) {
         // filter first, keeping the chunks, then the terms over them; the
         // list for the passes that follow is written on the way
-        constexpr uint32_t kKept = kLdsChunks + kSpillChunks;
         const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, nl, le, lst, cnt);
         auto load = [&](uint32_t j) { return cur[j]; };
         auto term = [&](uint32_t, const float4& q) { density_term(pi, q, sd, wsum, gx, gy, gz, gsq); };
-        visit_chunks(le, lst, min(K, kKept), s, nl, K <= kKept, load, term);
-        if (K > kKept)
-            neighbors_from_chunk(s, pi, sc, cur, cell_start, n, kKept, [&](uint32_t j) { term(j, load(j)); });
-        nl.count[s] = K <= kKept ? cnt : ~0u;
+        visit_chunks(
+            [&](uint32_t k) { return k < kLdsChunks ? le[k * lst] : nl.spill[(size_t)k * nl.stride + s]; },
+            min(K, kChunkScratch), s, nl, K <= kChunkScratch, load, term);
+        if (K > kChunkScratch)
+            neighbors_from_chunk(s, pi, sc, cur, cell_start, n, kChunkScratch,
+                                 [&](uint32_t j) { term(j, load(j)); });
+        // The list holds the neighbours when there are at most kMaxList and
+        // the walk fit the kept chunks (else visit_chunks did not list).
+        // More neighbours than the list holds, in kept chunks: the pass that
+        // follows takes them from the chunks, all of which go to the scratch.
+        // More chunks than that: the stencil.
+        uint32_t code = kCountStencil;
+        if (K <= kChunkScratch) {
+            code = cnt;
+            if (cnt > kMaxList) {
+                for (uint32_t k = 0; k < min(K, kLdsChunks); ++k) nl.spill[(size_t)k * nl.stride + s] = le[k * lst];
+                code = kCountChunks | K;
+            }
+        }
+        nl.count[s] = code;
     } else {
         for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
             cnt += two ? 2u : 1u;
@@ -423,7 +442,7 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
     const float4 pi = cur[s];
     const double lam_d = pi.w;
     double dx = 0.0, dy = 0.0, dz = 0.0;
-    for_each_neighbor_of<kList>(
+    for_each_neighbor_of<kList, true>(
         s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
         [&](uint32_t, const float4& q) { delta_term(pi, lam_d, q, sd, dx, dy, dz); });
     dx *= sd.inv_rho0_d;
