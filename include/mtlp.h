@@ -236,13 +236,14 @@ mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n
  * graph and replays it (the look-back words of the scan are cleared by a
  * kernel inside the graph).  Profiling is ignored while graphs are on. */
 mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
-/* Times eight steps at each stencil workgroup size from the current state
- * (restoring it afterwards) and keeps the fastest; with cache_path, reads a
- * previous choice for the same batch shape from that text file or appends
- * the new one.  *block receives the chosen size (may be NULL). */
+/* Times eight steps at each stencil workgroup size (64, 128, 256, 512)
+ * from the current state (restoring it afterwards) and keeps the fastest;
+ * with cache_path, reads a previous choice for the same batch shape from
+ * that text file or appends the new one.  *block receives the chosen size
+ * (may be NULL). */
 mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const char* cache_path,
                           uint32_t* block);
-/* Workgroup size of the stencil kernels (128, 256 or 512). */
+/* Workgroup size of the stencil kernels (64, 128, 256 or 512). */
 mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
 /* Neighbour lists (on by default): each iteration's density pass records
  * up to 64 neighbours per particle (260 B of device memory per particle)

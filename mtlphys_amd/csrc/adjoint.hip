@@ -608,6 +608,10 @@ __global__ __launch_bounds__(kBlock) void k_density_rho_vjp(const float4* __rest
 #define MTLP_STENCIL_LAUNCH(KERNEL, block, n, stream, ...)                                           \
     do {                                                                                             \
         switch (block) {                                                                             \
+            case 64:                                                                                 \
+                hipLaunchKernelGGL(KERNEL<64>, dim3(blocks_for(n, 64)), dim3(64), 0, stream,         \
+                                   __VA_ARGS__);                                                     \
+                break;                                                                               \
             case 128:                                                                                \
                 hipLaunchKernelGGL(KERNEL<128>, dim3(blocks_for(n, 128)), dim3(128), 0, stream,      \
                                    __VA_ARGS__);                                                     \
@@ -628,6 +632,7 @@ hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, c
                                       const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
+        case 64: density_lambda_tape_b<64>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
         case 128: density_lambda_tape_b<128>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
         case 512: density_lambda_tape_b<512>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;
         default: density_lambda_tape_b<256>(cur, cell_start, sorted_index, a, aux, lambda_orig, nbr_orig, nl, s); break;

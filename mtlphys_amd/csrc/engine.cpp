@@ -683,7 +683,7 @@ mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
     bool ok = false;
     for (uint32_t b : kStencilBlocks) ok |= b == block;
-    if (!ok) return fail(MTLP_E_INVALID, "stencil block must be 128, 256 or 512");
+    if (!ok) return fail(MTLP_E_INVALID, "stencil block must be 64, 128, 256 or 512");
     if (e->stencil_block != block) e->drop_graph();
     e->stencil_block = block;
     return MTLP_OK;

@@ -157,8 +157,8 @@ struct NbrList {
 };
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
-constexpr uint32_t kStencilBlocks[] = {128, 256, 512};
-constexpr uint32_t kNumStencilBlocks = 3;
+constexpr uint32_t kStencilBlocks[] = {64, 128, 256, 512};
+constexpr uint32_t kNumStencilBlocks = 4;
 
 // With nl.blk non-NULL the density pass writes the list and the delta pass
 // reads it.

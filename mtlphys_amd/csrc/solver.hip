@@ -5,8 +5,8 @@
 // :305-353) fused into one pass.  The per-slot code is in mtlp_stencil.h.
 //
 // Every kernel covers all scenes of a batch; a slot's scene constants come
-// from the scene table.  The stencil kernels are built for 128-, 256- and
-// 512-thread workgroups (the autotuner picks per batch).
+// from the scene table.  The stencil kernels are built for 64-, 128-, 256-
+// and 512-thread workgroups (the autotuner picks per batch).
 #include "mtlp_stencil.h"
 
 namespace mtlp {
@@ -169,6 +169,7 @@ hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
+        case 64: density_lambda_bl<64>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
         case 128: density_lambda_bl<128>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
         case 512: density_lambda_bl<512>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
         default: density_lambda_bl<256>(cur, cell_start, sorted_index, a, lambda_orig, nbr_orig, nl, s); break;
@@ -180,6 +181,7 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     switch (block) {
+        case 64: apply_delta_b<64>(cur, cell_start, a, next, nl, s); break;
         case 128: apply_delta_b<128>(cur, cell_start, a, next, nl, s); break;
         case 512: apply_delta_b<512>(cur, cell_start, a, next, nl, s); break;
         default: apply_delta_b<256>(cur, cell_start, a, next, nl, s); break;

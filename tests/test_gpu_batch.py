@@ -117,7 +117,7 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
     p = mtlp.scene_params(3, 20000)
     x, v = mtlp.generate_scene(3, 20000, 4, p)
     outs = {}
-    for blk in (128, 256, 512):
+    for blk in (64, 128, 256, 512):
         with mtlp.Engine(0, p) as e:
             e.reset(20000, mtlp.capi.SCENE_EMPTY)
             e.upload_state(x, v)
@@ -125,6 +125,7 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
             for _ in range(2):
                 e.step(DT)
             outs[blk] = e.positions_buffer().read()
+    np.testing.assert_array_equal(outs[64].view(np.uint32), outs[256].view(np.uint32))
     np.testing.assert_array_equal(outs[128].view(np.uint32), outs[256].view(np.uint32))
     np.testing.assert_array_equal(outs[512].view(np.uint32), outs[256].view(np.uint32))
     cache = tmp_path / "tune.txt"
@@ -133,7 +134,7 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
         e.upload_state(x, v)
         before = e.positions_buffer().read()
         blk = e.autotune(DT, cache)
-        assert blk in (128, 256, 512)
+        assert blk in (64, 128, 256, 512)
         np.testing.assert_array_equal(e.positions_buffer().read(), before)  # state restored
         assert e.autotune(DT, cache) == blk  # second call reads the cache
         assert len(cache.read_text().splitlines()) == 1
