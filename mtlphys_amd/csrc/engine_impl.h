@@ -143,7 +143,7 @@ struct mtlp_engine {
     uint32_t n_alloc = 0;
     uint32_t scene = MTLP_SCENE_CUBE_DROP;
     bool allocated = false;
-    uint32_t stencil_block = 256;
+    uint32_t stencil_block = 64;  // one wave per workgroup: fastest on every workload measured
 
     // scene table (device)
     SceneDev* d_scenes = nullptr;
