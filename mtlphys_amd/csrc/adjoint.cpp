@@ -99,6 +99,8 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
     t.lblk = A.lblk + (size_t)slot * (I ? I : 1) * kListBlocks * st;
     t.lcnt = A.lcnt + (size_t)slot * (I ? I : 1) * st;
     t.spill = e->nspill;
+    t.heavy = e->heavy;
+    t.heavy_count = e->heavy_count;
     t.stride = st;
     return t;
 }

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kB) void k_apply_delta_tape(const float4* __restric
                                                         float4* __restrict__ next, float4* __restrict__ rr,
                                                         NbrList nl) {
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-    if (s >= a.n) return;
+    if (!delta_takes_slot(s, a.n, nl)) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         apply_delta_slot<true, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
     });
@@ -644,7 +644,9 @@ hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start
                                    float4* next, float4* rr, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     MTLP_STENCIL_LAUNCH(k_apply_delta_tape, block, a.n, s, cur, cell_start, a, next, rr, nl);
-    return hipGetLastError();
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    return launch_delta_heavy(cur, cell_start, a, next, rr, nl, s);
 }
 
 hipError_t launch_finalize_tape(const float4* fin, const uint32_t* sorted_index, const StepArgs& a,

@@ -215,6 +215,9 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->nlist, (size_t)kListBlocks * na));
     HIP_TRY(dev_alloc(&e->ncount, na));
     HIP_TRY(dev_alloc(&e->nspill, (size_t)kChunkScratch * na));
+    HIP_TRY(dev_alloc(&e->heavy, 2 * (size_t)kHeavyCap));
+    HIP_TRY(dev_alloc(&e->heavy_count, 2));
+    HIP_TRY(hipMemset(e->heavy_count, 0, 2 * sizeof(uint32_t)));
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));
@@ -313,9 +316,9 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
         HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a, nullptr,
-                                      it == 0 ? e->nbr : nullptr, e->nbr_list(), e->stencil_block, s));
+                                      it == 0 ? e->nbr : nullptr, e->nbr_list(it), e->stencil_block, s));
         clk.mark("density_lambda");
-        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], e->nbr_list(),
+        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], e->nbr_list(it),
                                    e->stencil_block, s));
         clk.mark("apply_delta");
         cur ^= 1;

@@ -168,6 +168,8 @@ struct mtlp_engine {
     uint4* nlist = nullptr;
     uint32_t* ncount = nullptr;
     uint2* nspill = nullptr;  // kChunkScratch * n_alloc: density-pass chunk scratch
+    uint32_t* heavy = nullptr;        // 2 x kHeavyCap heavy-slot queues
+    uint32_t* heavy_count = nullptr;  // their 2 counters
     bool use_lists = true;
     // cell buffers
     uint32_t* counts = nullptr;
@@ -202,7 +204,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count);
         n_alloc = 0;
     }
     void free_cells() {
@@ -228,11 +230,15 @@ struct mtlp_engine {
         b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
+        b.heavy_count = heavy_count;
         return b;
     }
 
-    NbrList nbr_list() const {
-        return use_lists ? NbrList{nlist, ncount, n_alloc, nspill} : NbrList{nullptr, nullptr, 0u, nullptr};
+    // lists of solver iteration `it` (its heavy queue by parity)
+    NbrList nbr_list(uint32_t it) const {
+        if (!use_lists) return NbrList{};
+        return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
+                       heavy_count + ((it + 1u) & 1u)};
     }
 
     // Kinematic sphere of the sphere-drop scene (Engine.hpp:180-184),

@@ -106,6 +106,7 @@ struct HashBufs {
     uint32_t* big_count;
     uint32_t* err;
     uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
+    uint32_t* heavy_count; // 2 heavy-queue counters, cleared by the build; may be NULL
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------
@@ -147,13 +148,26 @@ constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
 constexpr uint32_t kLdsChunks = 16;
 constexpr uint32_t kChunkScratch = 32;
-constexpr uint32_t kCountChunks = 0x80000000u;
-constexpr uint32_t kCountStencil = 0xffffffffu;
+//
+// Such slots ("heavy": their walk is several times the typical one) also
+// go to a queue of up to kHeavyCap slots, marked kCountQueued in the count;
+// the delta pass leaves queued slots to a second launch that gives each a
+// whole wave (one lane alone would run long after the rest of the pass).
+// The queue is double-buffered by iteration parity: the delta pass of one
+// iteration clears the counter the next density pass appends to.
+constexpr uint32_t kCountChunks = 0x80000000u;   // | K: chunks in spill
+constexpr uint32_t kCountQueued = 0x40000000u;   // in the heavy queue
+constexpr uint32_t kCountStencil = kCountChunks | 0xffffu;
+constexpr uint32_t kHeavyCap = 4096;
+constexpr uint32_t kHeavyGrid = 2048;  // workgroups of the heavy launch (grid-stride over the queue)
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
     uint32_t stride;
     uint2* spill;     // kChunkScratch * stride: density-pass chunk scratch
+    uint32_t* heavy;             // kHeavyCap queued slots of this iteration
+    uint32_t* heavy_count;       // their number (may exceed kHeavyCap)
+    uint32_t* heavy_count_next;  // the next iteration's counter
 };
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
@@ -161,13 +175,17 @@ constexpr uint32_t kStencilBlocks[] = {64, 128, 256, 512};
 constexpr uint32_t kNumStencilBlocks = 4;
 
 // With nl.blk non-NULL the density pass writes the list and the delta pass
-// reads it.
+// reads it (launch_apply_delta then also runs launch_delta_heavy).
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  const uint32_t* sorted_index, const StepArgs& a,
                                  float* lambda_orig, uint32_t* nbr_orig, const NbrList& nl,
                                  uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
+// The heavy queue of a delta pass (rr: R before the cap, for the tape; may
+// be NULL).  Runs after the pass's main launch.
+hipError_t launch_delta_heavy(const float4* cur, const uint32_t* cell_start, const StepArgs& a, float4* next,
+                              float4* rr, const NbrList& nl, hipStream_t s);
 // Scatter back + finalize; with foam non-NULL also computeFoam from the
 // step's neighbour counts (nbr, original order).
 // Scatter back + finalize in original order: particle i takes its result
@@ -199,12 +217,15 @@ struct StepTape {
     uint4* lblk;             // I neighbour lists (kListBlocks * stride each)
     uint32_t* lcnt;          // I count arrays
     uint2* spill;            // the density pass's chunk scratch (not part of the tape)
+    uint32_t* heavy;         // 2 x kHeavyCap heavy-slot queues (iteration parity)
+    uint32_t* heavy_count;   // 2 counters
     size_t stride;
     float4* iterate(uint32_t k) const { return iter + k * stride; }
     float4* grad_sums(uint32_t k) const { return grad + k * stride; }
     float4* delta(uint32_t k) const { return dlt + k * stride; }
     NbrList list(uint32_t k) const {
-        return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride, spill};
+        return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride, spill,
+                       heavy + (k & 1u) * kHeavyCap, heavy_count + (k & 1u), heavy_count + ((k + 1u) & 1u)};
     }
 };
 

@@ -268,9 +268,10 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
             for_each_listed(s, cnt, nl, load, term);
             return;
         }
-        if (kScratch && (cnt & kCountChunks) && cnt != kCountStencil) {
-            visit_chunks([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, cnt & ~kCountChunks, s,
-                         nl, false, load, term);
+        const uint32_t K = cnt & 0xffffu;
+        if (kScratch && (cnt & kCountChunks) && K != 0xffffu) {
+            visit_chunks([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, K, s, nl, false, load,
+                         term);
             return;
         }
     }
@@ -322,30 +323,40 @@ __device__ __forceinline__ double ipow_d(double x, uint32_t n) {
 }
 
 // One neighbour's term of the position correction (Constraints.metal:171-181),
-// in FP64 from the FP32 positions and lambdas (all exact in FP64).
-__device__ __forceinline__ void delta_term(const float4& pi, double lam_i, const float4& pj,
-                                           const SolverDev& sd, double& dx, double& dy, double& dz) {
-    const double rx = (double)pi.x - (double)pj.x;
-    const double ry = (double)pi.y - (double)pj.y;
-    const double rz = (double)pi.z - (double)pj.z;
-    const double r2 = rx * rx + ry * ry + rz * rz;
+// in FP64 from the FP32 positions and lambdas (all exact in FP64): the
+// coefficient c and r = p_i - p_j, the term being c r.  Every fused
+// multiply-add is written out (contraction off), so each path that
+// computes the term (per lane, or a wave per slot) gets the same bits.
+struct DeltaTerm {
+    double c, rx, ry, rz;
+};
+
+__device__ __forceinline__ DeltaTerm delta_coef(const float4& pi, double lam_i, const float4& pj,
+                                                const SolverDev& sd) {
+#pragma clang fp contract(off)
+    DeltaTerm t;
+    t.rx = (double)pi.x - (double)pj.x;
+    t.ry = (double)pi.y - (double)pj.y;
+    t.rz = (double)pi.z - (double)pj.z;
+    const double r2 = fma(t.rz, t.rz, fma(t.rx, t.rx, t.ry * t.ry));
     // 1/r from the hardware estimate (~2^-23) plus one Newton step (~1e-14);
     // r2 = 0 gives NaN here and fails the range test below, like r <= 1e-6.
     const double y0 = __builtin_amdgcn_rsq(r2);
-    const double inv_r = y0 * (1.5 - 0.5 * r2 * y0 * y0);
+    const double inv_r = y0 * fma(y0, (-0.5 * r2) * y0, 1.5);
     const double rl = r2 * inv_r;
-    double sc = 0.0;
-    if (rl > (double)1e-6f && rl < sd.h_d) {
-        const double dd = sd.h_d - rl;
-        sc = -sd.spiky_d * dd * dd * inv_r;
-    }
+    const double dd = fma(-r2, inv_r, sd.h_d);
+    const double sc = rl > (double)1e-6f && rl < sd.h_d ? ((dd * -sd.spiky_d) * dd) * inv_r : 0.0;
     const double diff = sd.h2_d - r2;
-    const double wij = sd.poly6_d * diff * diff * diff;
-    const double scorr = -sd.scorr_k_d * ipow_d(wij * sd.inv_wq_d, sd.scorr_n);
-    const double coef = (lam_i + (double)pj.w + scorr) * sc;
-    dx += coef * rx;
-    dy += coef * ry;
-    dz += coef * rz;
+    const double wij = ((sd.poly6_d * diff) * diff) * diff;
+    const double lam = lam_i + (double)pj.w;
+    t.c = sc * fma(-sd.scorr_k_d, ipow_d(wij * sd.inv_wq_d, sd.scorr_n), lam);
+    return t;
+}
+
+__device__ __forceinline__ void add_term(const DeltaTerm& t, double& dx, double& dy, double& dz) {
+    dx = fma(t.c, t.rx, dx);
+    dy = fma(t.c, t.ry, dy);
+    dz = fma(t.c, t.rz, dz);
 }
 
 // densityLambdaSorted for slot s: lambda_s into cur[s].w.  With kTape the
@@ -395,6 +406,13 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
                 code = kCountChunks | K;
             }
         }
+        if (code > kMaxList) {  // heavy: queued for the front of the delta pass
+            const uint32_t q = atomicAdd(nl.heavy_count, 1u);
+            if (q < kHeavyCap) {
+                nl.heavy[q] = s;
+                code |= kCountQueued;
+            }
+        }
         nl.count[s] = code;
     } else {
         for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
@@ -417,11 +435,14 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     }
 }
 
-// applyDeltaSorted for slot s: next[s] = clamp(p_s + cap(R_s)) with
-// R_s = sum_j (lambda_s + lambda_j + s_corr) grad W / rho0, then the
-// sphere-drop scene's sphereCollision on next[s] (Engine.cpp:760-771).
-// With kTape R_s (before the 0.3 h cap) goes to rr[s] for the adjoint.  With
-// kList the neighbours come from the density pass's list.
+// Work item s of the delta pass: its slot unless the slot is queued as
+// heavy (then the heavy launch takes it).  Item 0 clears the counter of the
+// next iteration's queue.
+__device__ __forceinline__ bool delta_takes_slot(uint32_t s, uint32_t n, const NbrList& nl) {
+    if (s == 0) *nl.heavy_count_next = 0u;
+    return s < n && (nl.count[s] & kCountQueued) == 0u;
+}
+
 // sphereCollision (Constraints.metal:262-277): a point inside the sphere
 // moves to the surface along the ray from the centre.
 __device__ __forceinline__ float4 sphere_push(float4 p, const SphereDev& sp) {
@@ -432,24 +453,20 @@ __device__ __forceinline__ float4 sphere_push(float4 p, const SphereDev& sp) {
     return make_float4(sp.cx + dx / r * sp.r, sp.cy + dy / r * sp.r, sp.cz + dz / r * sp.r, 1.0f);
 }
 
-template <bool kTape, bool kList>
-__device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
-                                                 const float4* __restrict__ cur,
-                                                 const uint32_t* __restrict__ cell_start, uint32_t n,
-                                                 float4* __restrict__ next, float4* __restrict__ rr,
-                                                 const NbrList& nl, const SphereDev& sphere) {
-    const SolverDev& sd = sc.sd;
-    const float4 pi = cur[s];
-    const double lam_d = pi.w;
-    double dx = 0.0, dy = 0.0, dz = 0.0;
-    for_each_neighbor_of<kList, true>(
-        s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
-        [&](uint32_t, const float4& q) { delta_term(pi, lam_d, q, sd, dx, dy, dz); });
+// The end of applyDeltaSorted for slot s from R = the sum of its terms:
+// next[s] = clamp(p_s + cap(R / rho0)), then the sphere-drop scene's
+// sphereCollision (Engine.cpp:760-771).  With kTape R / rho0 (before the
+// 0.3 h cap) goes to rr[s] for the adjoint.
+template <bool kTape>
+__device__ __forceinline__ void finish_delta(uint32_t s, const float4& pi, const SolverDev& sd, double dx,
+                                             double dy, double dz, float4* __restrict__ next,
+                                             float4* __restrict__ rr, const SphereDev& sphere) {
+#pragma clang fp contract(off)
     dx *= sd.inv_rho0_d;
     dy *= sd.inv_rho0_d;
     dz *= sd.inv_rho0_d;
     if (kTape) rr[s] = make_float4((float)dx, (float)dy, (float)dz, 0.0f);
-    const double dlen = sqrt(dx * dx + dy * dy + dz * dz);
+    const double dlen = sqrt(fma(dz, dz, fma(dy, dy, dx * dx)));
     if (dlen > sd.max_step_d) {
         const double f = sd.max_step_d / dlen;
         dx *= f;
@@ -462,6 +479,91 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
                            (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
     if (sphere.on) q = sphere_push(q, sphere);
     next[s] = q;
+}
+
+// applyDeltaSorted for slot s (Constraints.metal:140-199): R_s = sum_j
+// (lambda_s + lambda_j + s_corr) grad W, summed in walk order.  With kList
+// the neighbours come from the density pass's list.
+template <bool kTape, bool kList>
+__device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
+                                                 const float4* __restrict__ cur,
+                                                 const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                 float4* __restrict__ next, float4* __restrict__ rr,
+                                                 const NbrList& nl, const SphereDev& sphere) {
+    const SolverDev& sd = sc.sd;
+    const float4 pi = cur[s];
+    const double lam_d = pi.w;
+    double dx = 0.0, dy = 0.0, dz = 0.0;
+    for_each_neighbor_of<kList, true>(
+        s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
+        [&](uint32_t, const float4& q) { add_term(delta_coef(pi, lam_d, q, sd), dx, dy, dz); });
+    finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
+}
+
+// applyDeltaSorted for a queued (heavy) slot s by a whole wave, the
+// workgroup: the lanes compute the terms of up to 64 neighbours at a time
+// (consecutive in walk order) into buf, and lane 0 adds them up in walk
+// order, so the bits are those of apply_delta_slot.  The neighbours come
+// from the chunks the density pass left in nl.spill, or from the stencil.
+constexpr uint32_t kHeavyBatch = 256;  // DeltaTerms in buf
+
+template <bool kTape>
+__device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc,
+                                                  const float4* __restrict__ cur,
+                                                  const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                  float4* __restrict__ next, float4* __restrict__ rr,
+                                                  const NbrList& nl, const SphereDev& sphere, DeltaTerm* buf) {
+    const SolverDev& sd = sc.sd;
+    const uint32_t lane = __lane_id();
+    const float4 pi = cur[s];
+    const double lam_d = pi.w;
+    double dx = 0.0, dy = 0.0, dz = 0.0;  // lane 0's sums
+    uint32_t nb = 0;                      // terms in buf
+    auto flush = [&]() {
+        __syncthreads();
+        if (lane == 0)
+            for (uint32_t i = 0; i < nb; ++i) add_term(buf[i], dx, dy, dz);
+        __syncthreads();
+        nb = 0;
+    };
+    // one group of candidates, lane l's being j (a neighbour when is_nbr);
+    // the neighbours' terms go to buf in lane order
+    auto group = [&](bool is_nbr, uint32_t j) {
+        const uint64_t bal = __ballot(is_nbr);
+        const uint32_t cnt = (uint32_t)__popcll(bal);
+        if (nb + cnt > kHeavyBatch) flush();
+        if (is_nbr) buf[nb + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = delta_coef(pi, lam_d, cur[j], sd);
+        nb += cnt;
+    };
+    const uint32_t K = nl.count[s] & 0xffffu;
+    if (K != 0xffffu) {
+        // chunks k and k + 1 per group: lanes 0-31 take chunk k's bits,
+        // lanes 32-63 chunk k + 1's, which is walk order
+        for (uint32_t k = 0; k < K; k += 2) {
+            const uint32_t kk = k + (lane >> 5);
+            const uint2 e = kk < K ? nl.spill[(size_t)kk * nl.stride + s] : make_uint2(0u, 0u);
+            const uint32_t bit = lane & 31u;
+            group(((e.y >> bit) & 1u) != 0u, e.x + bit);
+        }
+    } else {
+        // the stencil rows in walk order, 64 candidates per group, with the
+        // neighbour test of neighbor_mask
+        const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
+        const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
+        for (int k = 0; k < 9; ++k) {
+            for (uint32_t c0 = rows.a[k]; c0 < rows.b[k]; c0 += 64) {
+                const uint32_t j = c0 + lane;
+                bool is_nbr = false;
+                if (j < rows.b[k] && j != s) {
+                    const float4 q = cur[j];
+                    is_nbr = dist2(pi.x - q.x, pi.y - q.y, pi.z - q.z) - sd.h2 < 0.0f;
+                }
+                group(is_nbr, j);
+            }
+        }
+    }
+    flush();
+    if (lane == 0) finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
 }
 
 // finalizeStep (Constraints.metal:305-353) for one particle: pf is its
