@@ -283,29 +283,76 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
 }
 
 // One neighbour's contribution to the density sum (FP64) and to the
-// constraint gradient terms (Constraints.metal:110-119).  Every fused
-// multiply-add is written out (contraction off), so the result does not
-// depend on the context the compiler inlines this into.
-__device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
-                                             double& wsum, float& gx, float& gy, float& gz,
-                                             float& gsq) {
+// constraint gradient terms (Constraints.metal:110-119): its (h^2 - r^2)
+// and squared in FP64 and grad W in FP32, added to the sums in walk order
+// by add_density.  Every fused multiply-add is written out (contraction
+// off), so the bits do not depend on the context the compiler inlines this
+// into.
+struct DensityTerm {
+    double diff, diff2;
+    float ax, ay, az;
+};
+
+__device__ __forceinline__ DensityTerm density_pair(const float4& pi, const float4& pj, const SolverDev& sd) {
 #pragma clang fp contract(off)
+    DensityTerm t;
     const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
     const float r2 = dist2(rx, ry, rz);
     const double dxd = rx, dyd = ry, dzd = rz;
-    const double diff = sd.h2_d - fma(dzd, dzd, fma(dyd, dyd, dxd * dxd));
-    wsum = fma(diff, diff * diff, wsum);
+    t.diff = sd.h2_d - fma(dzd, dzd, fma(dyd, dyd, dxd * dxd));
+    t.diff2 = t.diff * t.diff;
     // gradient terms only enter lambda's denominator: hardware 1/sqrt (~1 ulp)
     const float inv_r = __builtin_amdgcn_rsqf(r2);
     const float rl = r2 * inv_r;
     // outside (1e-6, h) the term is zero (a select: r2 = 0 makes inv_r inf)
     const float dd = sd.h - rl;
     const float sc = rl > 1e-6f && rl < sd.h ? -sd.spiky * dd * dd * inv_r : 0.0f;
-    const float ax = sc * rx, ay = sc * ry, az = sc * rz;
-    gx += ax;
-    gy += ay;
-    gz += az;
-    gsq = fmaf(az, az, fmaf(ay, ay, fmaf(ax, ax, gsq)));
+    t.ax = sc * rx;
+    t.ay = sc * ry;
+    t.az = sc * rz;
+    return t;
+}
+
+__device__ __forceinline__ void add_density(const DensityTerm& t, double& wsum, float& gx, float& gy, float& gz,
+                                            float& gsq) {
+#pragma clang fp contract(off)
+    wsum = fma(t.diff, t.diff2, wsum);
+    gx += t.ax;
+    gy += t.ay;
+    gz += t.az;
+    gsq = fmaf(t.az, t.az, fmaf(t.ay, t.ay, fmaf(t.ax, t.ax, gsq)));
+}
+
+__device__ __forceinline__ void density_term(const float4& pi, const float4& pj, const SolverDev& sd,
+                                             double& wsum, float& gx, float& gy, float& gz, float& gsq) {
+    add_density(density_pair(pi, pj, sd), wsum, gx, gy, gz, gsq);
+}
+
+// lambda = -C / (D + eps) from the sums: C = rho / rho0 - 1 in FP64 with
+// rho = W(0) + poly6 sum, D = (|sum grad W|^2 + sum |grad W|^2) / rho0^2.
+// Returns lambda; dterm gets D.
+__device__ __forceinline__ float lambda_of(double wsum, float gx, float gy, float gz, float gsq,
+                                           const SolverDev& sd, float& dterm) {
+#pragma clang fp contract(off)
+    const double constraint = fma(sd.poly6_d, wsum, sd.w0_d) * sd.inv_rho0_d - 1.0;
+    dterm = (sd.inv_rho0 * sd.inv_rho0) * (fmaf(gz, gz, fmaf(gy, gy, gx * gx)) + gsq);
+    const float denom = dterm + sd.eps;
+    return (float)(-constraint / (double)denom);
+}
+
+// The list count code of a slot whose walk had K chunks and cnt neighbours
+// (see NbrList); a heavy slot also goes to the queue.
+__device__ __forceinline__ uint32_t list_code(uint32_t s, uint32_t K, uint32_t cnt, const NbrList& nl) {
+    uint32_t code = kCountStencil;
+    if (K <= kChunkScratch) code = cnt <= kMaxList ? cnt : kCountChunks | K;
+    if (code > kMaxList) {  // heavy: queued for the delta pass's wave-per-slot launch
+        const uint32_t q = atomicAdd(nl.heavy_count, 1u);
+        if (q < kHeavyCap) {
+            nl.heavy[q] = s;
+            code |= kCountQueued;
+        }
+    }
+    return code;
 }
 
 __device__ __forceinline__ double ipow_d(double x, uint32_t n) {
@@ -398,22 +445,9 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
         // More neighbours than the list holds, in kept chunks: the pass that
         // follows takes them from the chunks, all of which go to the scratch.
         // More chunks than that: the stencil.
-        uint32_t code = kCountStencil;
-        if (K <= kChunkScratch) {
-            code = cnt;
-            if (cnt > kMaxList) {
-                for (uint32_t k = 0; k < min(K, kLdsChunks); ++k) nl.spill[(size_t)k * nl.stride + s] = le[k * lst];
-                code = kCountChunks | K;
-            }
-        }
-        if (code > kMaxList) {  // heavy: queued for the front of the delta pass
-            const uint32_t q = atomicAdd(nl.heavy_count, 1u);
-            if (q < kHeavyCap) {
-                nl.heavy[q] = s;
-                code |= kCountQueued;
-            }
-        }
-        nl.count[s] = code;
+        if (K <= kChunkScratch && cnt > kMaxList)
+            for (uint32_t k = 0; k < min(K, kLdsChunks); ++k) nl.spill[(size_t)k * nl.stride + s] = le[k * lst];
+        nl.count[s] = list_code(s, K, cnt, nl);
     } else {
         for_each_neighbor(s, pi, sc, cur, cell_start, n, [&](uint32_t j1, uint32_t j2, bool two) {
             cnt += two ? 2u : 1u;
@@ -422,10 +456,8 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
             if (two) density_term(pi, q2, sd, wsum, gx, gy, gz, gsq);
         });
     }
-    const double constraint = (sd.w0_d + sd.poly6_d * wsum) * sd.inv_rho0_d - 1.0;
-    const float dterm = sd.inv_rho0 * sd.inv_rho0 * ((gx * gx + gy * gy + gz * gz) + gsq);
-    const float denom = dterm + sd.eps;
-    const float lambda = (float)(-constraint / (double)denom);
+    float dterm;
+    const float lambda = lambda_of(wsum, gx, gy, gz, gsq, sd, dterm);
     reinterpret_cast<float*>(cur)[4 * (size_t)s + 3] = lambda;
     if (kTape) aux[s] = make_float4(gx, gy, gz, dterm);
     if (kOrig || kCount) {
