@@ -78,11 +78,7 @@ __global__ __launch_bounds__(kB) void k_apply_delta_tape(const float4* __restric
                                                         const uint32_t* __restrict__ cell_start, StepArgs a,
                                                         float4* __restrict__ next, float4* __restrict__ rr,
                                                         NbrList nl) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-    if (!delta_takes_slot(s, a.n, nl)) return;
-    with_scene(s, a, [&](const SceneDev& sc) {
-        apply_delta_slot<true, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
-    });
+    delta_launch_body<kB, true>(cur, cell_start, a, next, rr, nl);
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize_tape(const float4* __restrict__ fin,
@@ -643,10 +639,25 @@ hipError_t launch_density_lambda_tape(float4* cur, const uint32_t* cell_start, c
 hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                                    float4* next, float4* rr, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    MTLP_STENCIL_LAUNCH(k_apply_delta_tape, block, a.n, s, cur, cell_start, a, next, rr, nl);
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    return launch_delta_heavy(cur, cell_start, a, next, rr, nl, s);
+    switch (block) {  // the heavy queue's workgroups in front
+        case 64:
+            hipLaunchKernelGGL(k_apply_delta_tape<64>, dim3(blocks_for(a.n, 64) + heavy_blocks<64>()), dim3(64), 0,
+                               s, cur, cell_start, a, next, rr, nl);
+            break;
+        case 128:
+            hipLaunchKernelGGL(k_apply_delta_tape<128>, dim3(blocks_for(a.n, 128) + heavy_blocks<128>()), dim3(128),
+                               0, s, cur, cell_start, a, next, rr, nl);
+            break;
+        case 512:
+            hipLaunchKernelGGL(k_apply_delta_tape<512>, dim3(blocks_for(a.n, 512) + heavy_blocks<512>()), dim3(512),
+                               0, s, cur, cell_start, a, next, rr, nl);
+            break;
+        default:
+            hipLaunchKernelGGL(k_apply_delta_tape<256>, dim3(blocks_for(a.n, 256) + heavy_blocks<256>()), dim3(256),
+                               0, s, cur, cell_start, a, next, rr, nl);
+            break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize_tape(const float4* fin, const uint32_t* sorted_index, const StepArgs& a,

@@ -151,15 +151,15 @@ constexpr uint32_t kChunkScratch = 32;
 //
 // Such slots ("heavy": their walk is several times the typical one) also
 // go to a queue of up to kHeavyCap slots, marked kCountQueued in the count;
-// the delta pass leaves queued slots to a second launch that gives each a
-// whole wave (one lane alone would run long after the rest of the pass).
+// the delta pass's first workgroups give each queued slot a whole wave (one
+// lane alone would run long after the rest of the pass) and the others skip
+// them.
 // The queue is double-buffered by iteration parity: the delta pass of one
 // iteration clears the counter the next density pass appends to.
 constexpr uint32_t kCountChunks = 0x80000000u;   // | K: chunks in spill
 constexpr uint32_t kCountQueued = 0x40000000u;   // in the heavy queue
 constexpr uint32_t kCountStencil = kCountChunks | 0xffffu;
 constexpr uint32_t kHeavyCap = 4096;
-constexpr uint32_t kHeavyGrid = 2048;  // workgroups of the heavy launch (grid-stride over the queue)
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
@@ -175,17 +175,13 @@ constexpr uint32_t kStencilBlocks[] = {64, 128, 256, 512};
 constexpr uint32_t kNumStencilBlocks = 4;
 
 // With nl.blk non-NULL the density pass writes the list and the delta pass
-// reads it (launch_apply_delta then also runs launch_delta_heavy).
+// reads it.
 hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  const uint32_t* sorted_index, const StepArgs& a,
                                  float* lambda_orig, uint32_t* nbr_orig, const NbrList& nl,
                                  uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
-// The heavy queue of a delta pass (rr: R before the cap, for the tape; may
-// be NULL).  Runs after the pass's main launch.
-hipError_t launch_delta_heavy(const float4* cur, const uint32_t* cell_start, const StepArgs& a, float4* next,
-                              float4* rr, const NbrList& nl, hipStream_t s);
 // Scatter back + finalize; with foam non-NULL also computeFoam from the
 // step's neighbour counts (nbr, original order).
 // Scatter back + finalize in original order: particle i takes its result

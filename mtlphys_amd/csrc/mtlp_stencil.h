@@ -532,40 +532,39 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
     finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
 }
 
-// applyDeltaSorted for a queued (heavy) slot s by a whole wave, the
-// workgroup: the lanes compute the terms of up to 64 neighbours at a time
-// (consecutive in walk order) into buf, and lane 0 adds them up in walk
-// order, so the bits are those of apply_delta_slot.  The neighbours come
-// from the chunks the density pass left in nl.spill, or from the stencil.
-constexpr uint32_t kHeavyBatch = 256;  // DeltaTerms in buf
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
+// applyDeltaSorted for a queued (heavy) slot s by a whole wave: the lanes
+// compute the terms of up to 64 neighbours at a time (consecutive in walk
+// order, in lane order), and the wave adds them up one by one in walk
+// order (each term read across lanes into every lane), so the bits are
+// those of apply_delta_slot.  The neighbours come from the chunks the
+// density pass left in nl.spill, or from the stencil.  s is wave-uniform.
 template <bool kTape>
 __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
                                                   float4* __restrict__ next, float4* __restrict__ rr,
-                                                  const NbrList& nl, const SphereDev& sphere, DeltaTerm* buf) {
+                                                  const NbrList& nl, const SphereDev& sphere) {
     const SolverDev& sd = sc.sd;
     const uint32_t lane = __lane_id();
     const float4 pi = cur[s];
     const double lam_d = pi.w;
-    double dx = 0.0, dy = 0.0, dz = 0.0;  // lane 0's sums
-    uint32_t nb = 0;                      // terms in buf
-    auto flush = [&]() {
-        __syncthreads();
-        if (lane == 0)
-            for (uint32_t i = 0; i < nb; ++i) add_term(buf[i], dx, dy, dz);
-        __syncthreads();
-        nb = 0;
-    };
-    // one group of candidates, lane l's being j (a neighbour when is_nbr);
-    // the neighbours' terms go to buf in lane order
+    double dx = 0.0, dy = 0.0, dz = 0.0;  // the same in every lane
+    // one group of candidates, lane l's being j (a neighbour when is_nbr)
     auto group = [&](bool is_nbr, uint32_t j) {
-        const uint64_t bal = __ballot(is_nbr);
-        const uint32_t cnt = (uint32_t)__popcll(bal);
-        if (nb + cnt > kHeavyBatch) flush();
-        if (is_nbr) buf[nb + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = delta_coef(pi, lam_d, cur[j], sd);
-        nb += cnt;
+        DeltaTerm t{0.0, 0.0, 0.0, 0.0};
+        if (is_nbr) t = delta_coef(pi, lam_d, cur[j], sd);
+        for (uint64_t bal = __ballot(is_nbr); bal; bal &= bal - 1ull) {
+            const int l = __builtin_ctzll(bal);
+            add_term(DeltaTerm{readlane_d(t.c, l), readlane_d(t.rx, l), readlane_d(t.ry, l), readlane_d(t.rz, l)},
+                     dx, dy, dz);
+        }
     };
     const uint32_t K = nl.count[s] & 0xffffu;
     if (K != 0xffffu) {
@@ -594,8 +593,42 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
             }
         }
     }
-    flush();
     if (lane == 0) finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
+}
+
+// A delta launch over n slots with lists: workgroups [0, heavy_blocks<kB>)
+// come first in dispatch order and run the heavy queue, a wave per queued
+// slot (grid-stride over the queue, kHeavyWaves waves); the others take
+// the slots.  Returns the workgroups for n slots.
+constexpr uint32_t kHeavyWaves = 512;
+
+template <uint32_t kB>
+constexpr uint32_t heavy_blocks() {
+    return kHeavyWaves / (kB / 64u);
+}
+
+template <uint32_t kB, bool kTape>
+__device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur,
+                                                  const uint32_t* __restrict__ cell_start, const StepArgs& a,
+                                                  float4* __restrict__ next, float4* __restrict__ rr,
+                                                  const NbrList& nl) {
+    constexpr uint32_t hb = heavy_blocks<kB>();
+    if (blockIdx.x < hb) {
+        const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kB / 64u) + threadIdx.x / 64u));
+        const uint32_t h_end = min(*nl.heavy_count, kHeavyCap);
+        for (uint32_t h = wave; h < h_end; h += kHeavyWaves) {
+            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)nl.heavy[h]);
+            with_scene(s, a, [&](const SceneDev& sc) {
+                apply_delta_heavy<kTape>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
+            });
+        }
+        return;
+    }
+    const uint32_t s = xcd_block(blockIdx.x - hb, gridDim.x - hb) * kB + threadIdx.x;
+    if (!delta_takes_slot(s, a.n, nl)) return;
+    with_scene(s, a, [&](const SceneDev& sc) {
+        apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
+    });
 }
 
 // finalizeStep (Constraints.metal:305-353) for one particle: pf is its

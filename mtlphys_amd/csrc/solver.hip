@@ -46,29 +46,15 @@ template <uint32_t kB, bool kList>
 __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ cur,
                                                    const uint32_t* __restrict__ cell_start,
                                                    StepArgs a, float4* __restrict__ next, NbrList nl) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-    const uint32_t n = a.n;
-    if (kList ? !delta_takes_slot(s, n, nl) : s >= n) return;
-    with_scene(s, a, [&](const SceneDev& sc) {
-        apply_delta_slot<false, kList>(s, sc, cur, cell_start, n, next, nullptr, nl, a.sphere);
-    });
-}
-
-// The delta pass's heavy queue: one wave (the workgroup) per queued slot,
-// a grid-stride loop over the queue.
-template <bool kTape>
-__global__ __launch_bounds__(64) void k_apply_delta_heavy(const float4* __restrict__ cur,
-                                                         const uint32_t* __restrict__ cell_start, StepArgs a,
-                                                         float4* __restrict__ next, float4* __restrict__ rr,
-                                                         NbrList nl) {
-    __shared__ DeltaTerm buf[kHeavyBatch];
-    const uint32_t h_end = min(*nl.heavy_count, kHeavyCap);
-    for (uint32_t h = blockIdx.x; h < h_end; h += gridDim.x) {
-        const uint32_t s = nl.heavy[h];
-        with_scene(s, a, [&](const SceneDev& sc) {
-            apply_delta_heavy<kTape>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere, buf);
-        });
+    if (kList) {
+        delta_launch_body<kB, false>(cur, cell_start, a, next, nullptr, nl);
+        return;
     }
+    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    if (s >= a.n) return;
+    with_scene(s, a, [&](const SceneDev& sc) {
+        apply_delta_slot<false, false>(s, sc, cur, cell_start, a.n, next, nullptr, nl, a.sphere);
+    });
 }
 
 // Scatter back + finalize (+ computeFoam), in original order: particle i
@@ -173,7 +159,8 @@ void density_lambda_bl(float4* cur, const uint32_t* cs, const uint32_t* sidx, co
 template <uint32_t kB>
 void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
                    hipStream_t s) {
-    const dim3 grid(blocks_for(a.n, kB)), block(kB);
+    // with lists, the heavy queue's workgroups in front
+    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() : 0u)), block(kB);
     if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
     else hipLaunchKernelGGL((k_apply_delta<kB, false>), grid, block, 0, s, cur, cs, a, next, nl);
 }
@@ -194,14 +181,6 @@ hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
     return hipGetLastError();
 }
 
-hipError_t launch_delta_heavy(const float4* cur, const uint32_t* cell_start, const StepArgs& a, float4* next,
-                              float4* rr, const NbrList& nl, hipStream_t s) {
-    if (a.n == 0 || !nl.blk) return hipSuccess;
-    if (rr) hipLaunchKernelGGL(k_apply_delta_heavy<true>, dim3(kHeavyGrid), dim3(64), 0, s, cur, cell_start, a, next, rr, nl);
-    else hipLaunchKernelGGL(k_apply_delta_heavy<false>, dim3(kHeavyGrid), dim3(64), 0, s, cur, cell_start, a, next, rr, nl);
-    return hipGetLastError();
-}
-
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
@@ -211,9 +190,7 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
         case 512: apply_delta_b<512>(cur, cell_start, a, next, nl, s); break;
         default: apply_delta_b<256>(cur, cell_start, a, next, nl, s); break;
     }
-    const hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    return launch_delta_heavy(cur, cell_start, a, next, nullptr, nl, s);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
