@@ -332,7 +332,8 @@ def test_dense_regions_chunk_paths(mtlp, orc):
     """Blobs of three densities drive the density pass through every way it
     keeps a slot's stencil chunks: all in LDS (<= 16), the global scratch
     (17..32, with and without a neighbour list of <= 96 entries), and the
-    second stencil walk (> 32).  Lists on and off must agree bit for bit,
+    second stencil walk (> 32).  Lists on and off must agree bit for bit
+    over two steps (the second starts from marks the first left, if any),
     and each iteration's kernels must match the FP64 oracle."""
     n = 20000
     p = mtlp.scene_params(3, n)
@@ -356,8 +357,10 @@ def test_dense_regions_chunk_paths(mtlp, orc):
             e.reset(n, mtlp.capi.SCENE_EMPTY)
             e.upload_state(x, v)
             e.set_neighbor_lists(lists)
-            e.step(dt)
-            outs[lists] = (e.positions_buffer().read(), e.velocities_buffer().read(), e.lambdas_buffer().read())
+            outs[lists] = []
+            for _ in range(2):
+                e.step(dt)
+                outs[lists] += [e.positions_buffer().read(), e.velocities_buffer().read(), e.lambdas_buffer().read()]
             assert e.error_flags() == 0
     for a, b in zip(outs[False], outs[True]):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
