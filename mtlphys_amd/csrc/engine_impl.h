@@ -158,7 +158,7 @@ struct mtlp_engine {
     float* lambdas = nullptr;
     float* foam = nullptr;
     // particle buffers (sorted order)
-    uint32_t* tmp_index = nullptr;
+    uint2* tmp_index = nullptr;  // (index, key) of the hash's unstable scatter
     uint32_t* sorted_index = nullptr;
     uint32_t* slot_of = nullptr;  // sorted slot of each particle
     float4* sorted[2] = {nullptr, nullptr};
@@ -227,7 +227,8 @@ struct mtlp_engine {
     HashBufs hash_bufs() {
         HashBufs b{};
         b.x = x; b.v = v; b.keys = keys; b.tickets = tickets; b.counts = counts;
-        b.cell_start = cell_start; b.tmp_index = tmp_index; b.sorted_index = sorted_index;
+        b.cell_start = cell_start; b.tmp = tmp_index; b.sorted_index = sorted_index;
+        b.stage = sorted[1];  // the second iterate buffer is free until the first delta pass
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;

@@ -7,14 +7,16 @@
 // hashCells, countCells, [scan], scatterParticles, gatherPositions,
 // countNeighbors.  Differences by design:
 //   * predict, hash and count are one pass over x and v; the predicted
-//     position is not stored (the gather recomputes it bit for bit);
+//     position is not stored (the scatter recomputes it bit for bit);
 //   * the within-cell order is stable by original index.  The count atomic
 //     returns each particle's arrival rank ("ticket"); after the scan the
 //     particle lands at cell_start + ticket (unordered, no second atomic
-//     pass), then every slot ranks its index among its cell's indices and
-//     moves to cell_start + rank, fetching its position on the way.  Cells
-//     larger than kSmallCell are listed and sorted by a separate kernel
-//     (LDS bitonic sort, or an ordered key scan past kLdsSortMax).
+//     pass) with its key and predicted position, read in index order; then
+//     every slot ranks its index among its cell's indices and moves its
+//     entry to cell_start + rank, so the random accesses of the pair are
+//     the scatter's stores rather than gathers.  Cells larger than
+//     kSmallCell are listed and sorted by a separate kernel (LDS bitonic
+//     sort, or an ordered key scan past kLdsSortMax).
 #include "mtlp_device.h"
 
 namespace mtlp {
@@ -42,33 +44,40 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
 
 // Offsets and tickets are consistent by construction (counts start at
 // zero); the range checks only turn a broken invariant into an error flag
-// instead of an out-of-bounds access.
+// instead of an out-of-bounds access.  Each particle lands at cell_start +
+// ticket with its key and binned position, read here in index order, so
+// the ranking pass that follows reads its slot's data in slot order.
 __global__ __launch_bounds__(kBlock) void k_scatter_ticket(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ tickets,
-    const uint32_t* __restrict__ cell_start, uint32_t n, uint32_t* __restrict__ tmp,
+    const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
+    const float4* __restrict__ v, StepArgs a, uint2* __restrict__ tmp, float4* __restrict__ stage,
     uint32_t* __restrict__ err) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = a.n;
     if (i >= n) return;
-    const uint32_t d = cell_start[keys[i]] + tickets[i];
-    if (d < n) tmp[d] = i;
-    else atomicOr(err, kErrHashRange);
+    const uint32_t key = keys[i];
+    const uint32_t d = cell_start[key] + tickets[i];
+    if (d < n) {
+        tmp[d] = make_uint2(i, key);
+        stage[d] = binned_pos(x, v, i, a);
+    } else {
+        atomicOr(err, kErrHashRange);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
-    const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ keys,
-    const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
-    const float4* __restrict__ v, StepArgs a, uint32_t* __restrict__ sorted_index,
+    const uint2* __restrict__ tmp, const float4* __restrict__ stage,
+    const uint32_t* __restrict__ cell_start, uint32_t n, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted, uint32_t* __restrict__ big_list, uint32_t* __restrict__ big_count,
     uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t n = a.n;
     if (s >= n) return;
-    const uint32_t i = tmp[s];
+    const uint2 e = tmp[s];
+    const uint32_t i = e.x, key = e.y;
     if (i >= n) {
         atomicOr(err, kErrHashRange);
         return;
     }
-    const uint32_t key = keys[i];
     const uint32_t lo = cell_start[key], hi = cell_start[key + 1];
     if (hi > n || lo > hi) {
         atomicOr(err, kErrHashRange);
@@ -79,16 +88,16 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
         return;
     }
     uint32_t rank = 0;
-    for (uint32_t m = lo; m < hi; ++m) rank += (tmp[m] < i) ? 1u : 0u;
+    for (uint32_t m = lo; m < hi; ++m) rank += (tmp[m].x < i) ? 1u : 0u;
     const uint32_t dst = lo + rank;
     sorted_index[dst] = i;
-    sorted[dst] = binned_pos(x, v, i, a);
+    sorted[dst] = stage[s];
     if (slot_of) slot_of[i] = dst;
 }
 
 // One workgroup per listed cell (grid-stride over the list).
 __global__ __launch_bounds__(kBigBlock) void k_big_cells(
-    const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ keys,
+    const uint2* __restrict__ tmp, const uint32_t* __restrict__ keys,
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, const uint32_t* __restrict__ big_list,
     const uint32_t* __restrict__ big_count, uint32_t* __restrict__ sorted_index,
@@ -108,7 +117,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
             uint32_t p2 = 1;
             while (p2 < len) p2 <<= 1;
             for (uint32_t k = t; k < p2; k += kBigBlock) {
-                const uint32_t idx = (k < len) ? tmp[lo + k] : 0xffffffffu;
+                const uint32_t idx = (k < len) ? tmp[lo + k].x : 0xffffffffu;
                 lds[k] = (k < len && idx >= n) ? 0u : idx;  // out-of-range ids were flagged
             }
             __syncthreads();
@@ -225,14 +234,13 @@ hipError_t launch_scatter_gather(const HashBufs& b, const StepArgs& a, hipStream
     if (a.n == 0) return hipSuccess;
     const uint32_t grid = blocks_for(a.n, kBlock);
     hipLaunchKernelGGL(k_scatter_ticket, dim3(grid), dim3(kBlock), 0, s, b.keys, b.tickets,
-                       b.cell_start, a.n, b.tmp_index, b.err);
-    hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp_index, b.keys,
-                       b.cell_start, b.x, b.v, a, b.sorted_index, b.sorted, b.big_list,
-                       b.big_count, b.err, b.slot_of);
+                       b.cell_start, b.x, b.v, a, b.tmp, b.stage, b.err);
+    hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp, b.stage, b.cell_start, a.n,
+                       b.sorted_index, b.sorted, b.big_list, b.big_count, b.err, b.slot_of);
     // Only cells above kSmallCell particles exist past this point; with
     // n <= kSmallCell there can be none.
     if (a.n > kSmallCell)
-        hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp_index, b.keys,
+        hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp, b.keys,
                            b.cell_start, b.x, b.v, a, b.big_list, b.big_count, b.sorted_index,
                            b.sorted, b.slot_of);
     return hipGetLastError();

@@ -99,7 +99,8 @@ struct HashBufs {
     uint32_t* tickets;    // arrival rank within the cell from the count atomics
     uint32_t* counts;     // C + 1 counters, all zero on entry, zero on exit
     uint32_t* cell_start; // C + 1
-    uint32_t* tmp_index;  // unordered cell-grouped indices
+    uint2* tmp;           // (index, key) grouped by cell, unordered within a cell
+    float4* stage;        // binned positions in tmp's order (may be a free iterate buffer)
     uint32_t* sorted_index;
     float4* sorted;       // gathered (predicted) positions
     uint32_t* big_list;   // cells larger than kSmallCell
