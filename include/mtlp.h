@@ -236,8 +236,9 @@ mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n
  * graph and replays it (the look-back words of the scan are cleared by a
  * kernel inside the graph).  Profiling is ignored while graphs are on. */
 mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
-/* Times eight steps at each stencil workgroup size (64, 128, 256, 512)
- * from the current state (restoring it afterwards) and keeps the fastest;
+/* Times eight steps at each stencil workgroup size (64, 128, 256, 512),
+ * two passes over the sizes, from the current state (restoring it
+ * afterwards) and keeps the size with the fastest pass;
  * with cache_path, reads a previous choice for the same batch shape from
  * that text file or appends the new one.  *block receives the chosen size
  * (may be NULL). */
@@ -246,9 +247,10 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
 /* Workgroup size of the stencil kernels (64, 128, 256 or 512). */
 mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
 /* Neighbour lists (on by default): each iteration's density pass records
- * up to 64 neighbours per particle (260 B of device memory per particle)
- * and the delta pass walks them instead of testing the 27-cell stencil's
- * candidates again.  Results are bit-identical with lists on or off. */
+ * up to 96 neighbours per particle, or the stencil chunks that hold them
+ * (644 B of device memory per particle), and the delta pass walks them
+ * instead of testing the 27-cell stencil's candidates again.  Results are
+ * bit-identical with lists on or off. */
 mtlp_status mtlp_set_neighbor_lists(mtlp_engine* e, int enable);
 
 /* ---- state I/O (no reference counterpart) ------------------------------
