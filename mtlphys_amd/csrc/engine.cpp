@@ -219,6 +219,11 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->heavy, 2 * (size_t)kHeavyCap));
     HIP_TRY(dev_alloc(&e->heavy_count, 2));
     HIP_TRY(hipMemset(e->heavy_count, 0, 2 * sizeof(uint32_t)));
+    HIP_TRY(dev_alloc(&e->tile_plan, (na + 63) / 64));
+    HIP_TRY(dev_alloc(&e->tile_order, (na + 63) / 64));
+    HIP_TRY(dev_alloc(&e->tile_count, kTileCounters));
+    HIP_TRY(hipMemset(e->tile_count, 0, kTileCounters * sizeof(uint32_t)));
+    e->tile_order_tiles = 0;
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));
@@ -324,6 +329,8 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
         clk.mark("apply_delta");
         cur ^= 1;
     }
+    // the delta passes wrote the next density pass's tile order
+    if (e->use_lists && iters > 0) e->tile_order_tiles = e->stencil_tiles();
     if (iters == 0) {
         HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
         clk.mark("count_neighbors");

@@ -170,6 +170,10 @@ struct mtlp_engine {
     uint2* nspill = nullptr;  // kChunkScratch * n_alloc: density-pass chunk scratch
     uint32_t* heavy = nullptr;        // 2 x kHeavyCap heavy-slot queues
     uint32_t* heavy_count = nullptr;  // their 2 counters
+    uint32_t* tile_plan = nullptr;    // ceil(n_alloc / 64): density-pass tile costs (NbrList::tile_plan)
+    uint32_t* tile_count = nullptr;   // kTileCounters
+    uint32_t* tile_order = nullptr;   // ceil(n_alloc / 64): the next density pass's tile order
+    uint32_t tile_order_tiles = 0;    // tile_order holds a permutation of this many tiles (0: none)
     bool use_lists = true;
     // cell buffers
     uint32_t* counts = nullptr;
@@ -204,7 +208,8 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(tile_plan); dev_free(tile_count); dev_free(tile_order);
+        tile_order_tiles = 0;
         n_alloc = 0;
     }
     void free_cells() {
@@ -232,14 +237,20 @@ struct mtlp_engine {
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;
+        b.tile_count = tile_count;
         return b;
     }
 
-    // lists of solver iteration `it` (its heavy queue by parity)
+    uint32_t stencil_tiles() const { return (n + stencil_block - 1) / stencil_block; }
+
+    // lists of solver iteration `it` (its heavy queue and tile counters by
+    // parity); the density pass follows tile_order when it is a permutation
+    // of this launch's tiles
     NbrList nbr_list(uint32_t it) const {
         if (!use_lists) return NbrList{};
         return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
-                       heavy_count + ((it + 1u) & 1u)};
+                       heavy_count + ((it + 1u) & 1u), tile_plan, tile_count, it & 1u,
+                       tile_order_tiles == stencil_tiles() ? tile_order : nullptr, tile_order};
     }
 
     // Kinematic sphere of the sphere-drop scene (Engine.hpp:180-184),

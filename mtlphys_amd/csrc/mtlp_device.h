@@ -160,4 +160,17 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return x * q + min(x, r) + i;
 }
 
+// First position of XCD x's range under xcd_block, and the XCD whose range
+// holds position t.
+__device__ __forceinline__ uint32_t xcd_range_start(uint32_t x, uint32_t nb) {
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    return x * q + min(x, r);
+}
+__device__ __forceinline__ uint32_t xcd_of_position(uint32_t t, uint32_t nb) {
+    const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+    const uint32_t big = r * (q + 1u);  // the first r ranges hold q + 1 positions
+    return t < big ? t / (q + 1u) : r + (t - big) / q;  // q > 0 whenever t >= big
+}
+static_assert(kTileCounters == 2 * kNumXcd * kCostBuckets, "tile counters per XCD");
+
 }  // namespace mtlp

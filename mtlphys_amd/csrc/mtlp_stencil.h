@@ -607,6 +607,48 @@ constexpr uint32_t heavy_blocks() {
     return kHeavyWaves / (kB / 64u);
 }
 
+// Times a stencil workgroup's tile (its first wave, from the start to the
+// exit of thread 0) and ranks the tile in its XCD's cost bucket, for the
+// order of the next density pass (NbrList::tile_order).
+struct TileTimer {
+    uint64_t t0;
+    uint32_t* plan;
+    uint32_t* count;
+    uint32_t tile, nb;
+    __device__ TileTimer(const NbrList& nl, uint32_t tile_, uint32_t nb_)
+        : t0(__builtin_amdgcn_s_memrealtime()), plan(nl.tile_plan),
+          count(nl.tile_plan ? nl.tile_count + nl.tile_parity * (kTileCounters / 2) : nullptr), tile(tile_), nb(nb_) {}
+    __device__ ~TileTimer() {
+        if (plan && threadIdx.x == 0) {
+            const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t0;
+            const uint32_t b = (uint32_t)min(dt / kCostTicks, (uint64_t)(kCostBuckets - 1u));
+            const uint32_t rank = atomicAdd(count + xcd_of_position(tile, nb) * kCostBuckets + b, 1u);
+            plan[tile] = b << 24 | rank;
+        }
+    }
+};
+
+// The delta pass's wave 0 of tile `tile` (of nb): the tile's position in
+// the next density pass's order, heavier cost buckets of its XCD's range
+// first; the workgroup of tile 0 clears the counters that pass will use.
+// Runs with the whole wave active.
+__device__ __forceinline__ void order_tile(const NbrList& nl, uint32_t tile, uint32_t nb) {
+    const uint32_t lane = __lane_id();
+    const uint32_t* cnt = nl.tile_count + nl.tile_parity * (kTileCounters / 2);
+    const uint32_t e = nl.tile_plan[tile];
+    const uint32_t b = e >> 24, rank = e & 0xffffffu;
+    const uint32_t x = xcd_of_position(tile, nb);
+    const uint32_t c = lane < kCostBuckets && lane > b ? cnt[x * kCostBuckets + lane] : 0u;
+    const uint32_t heavier = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(c), 63);
+    if (lane == 0) {
+        const uint32_t p = xcd_range_start(x, nb) + heavier + rank;
+        if (p < nb) nl.tile_order_next[p] = tile;
+    }
+    if (tile == 0)
+        for (uint32_t k = lane; k < kTileCounters / 2; k += 64u)
+            nl.tile_count[(nl.tile_parity ^ 1u) * (kTileCounters / 2) + k] = 0u;
+}
+
 template <uint32_t kB, bool kTape>
 __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, const StepArgs& a,
@@ -624,7 +666,10 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
         }
         return;
     }
-    const uint32_t s = xcd_block(blockIdx.x - hb, gridDim.x - hb) * kB + threadIdx.x;
+    const uint32_t nbt = gridDim.x - hb;
+    const uint32_t tile = xcd_block(blockIdx.x - hb, nbt);
+    if (nl.tile_order_next && threadIdx.x < 64u) order_tile(nl, tile, nbt);
+    const uint32_t s = tile * kB + threadIdx.x;
     if (!delta_takes_slot(s, a.n, nl)) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);

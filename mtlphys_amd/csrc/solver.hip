@@ -19,7 +19,14 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
     uint32_t* __restrict__ nbr_orig, NbrList nl) {
     __shared__ uint2 ent[kList ? kLdsChunks * kB : 1];  // each thread's recorded chunks
-    const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t nb = gridDim.x;
+    const uint32_t pos = xcd_block(blockIdx.x, nb);
+    // the tile (kB slots) this workgroup takes: in the order the last delta
+    // pass left (heaviest first within each XCD's range), else in place
+    const uint32_t tile =
+        kList && nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nb - 1u) : pos;
+    const TileTimer timer(kList ? nl : NbrList{}, tile, nb);
+    const uint32_t s0 = tile * kB + threadIdx.x;
     const uint32_t n = a.n;
     // The wave's slots are handed to its lanes in order of their current
     // cell, so lanes that walk the same stencil rows sit together: their
