@@ -220,10 +220,11 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->heavy_count, 2));
     HIP_TRY(hipMemset(e->heavy_count, 0, 2 * sizeof(uint32_t)));
     HIP_TRY(dev_alloc(&e->tile_plan, (na + 63) / 64));
-    HIP_TRY(dev_alloc(&e->tile_order, (na + 63) / 64));
+    HIP_TRY(dev_alloc(&e->tile_order[0], (na + 63) / 64));
+    HIP_TRY(dev_alloc(&e->tile_order[1], (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_count, kTileCounters));
     HIP_TRY(hipMemset(e->tile_count, 0, kTileCounters * sizeof(uint32_t)));
-    e->tile_order_tiles = 0;
+    e->tile_order_tiles[0] = e->tile_order_tiles[1] = 0;
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));
@@ -321,16 +322,15 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     const uint32_t iters = e->iterations();
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
+        const NbrList nl = e->nbr_list(it);
         HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a, nullptr,
-                                      it == 0 ? e->nbr : nullptr, e->nbr_list(it), e->stencil_block, s));
+                                      it == 0 ? e->nbr : nullptr, nl, e->stencil_block, s));
         clk.mark("density_lambda");
-        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], e->nbr_list(it),
-                                   e->stencil_block, s));
+        HIP_TRY(launch_apply_delta(e->sorted[cur], e->cell_start, a, e->sorted[cur ^ 1], nl, e->stencil_block, s));
         clk.mark("apply_delta");
+        if (nl.blk) e->flip_tile_order();
         cur ^= 1;
     }
-    // the delta passes wrote the next density pass's tile order
-    if (e->use_lists && iters > 0) e->tile_order_tiles = e->stencil_tiles();
     if (iters == 0) {
         HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
         clk.mark("count_neighbors");

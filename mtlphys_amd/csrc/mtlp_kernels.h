@@ -175,13 +175,13 @@ struct NbrList {
     // Tile order: each density pass times its tiles (one stencil workgroup
     // of slots each) and ranks them by cost within their XCD's range; the
     // delta pass that follows turns the ranks into the order in which the
-    // next density pass takes them, heaviest first, so that its last
+    // next iteration's passes take them, heaviest first, so that their last
     // workgroups are short ones.  The results do not depend on the order.
     uint32_t* tile_plan;        // per tile: cost bucket << 24 | rank in (XCD, bucket)
     uint32_t* tile_count;       // 2 x kNumXcd x kCostBuckets counters, by iteration parity
     uint32_t tile_parity;
-    const uint32_t* tile_order; // density pass: tile at each position (NULL: identity)
-    uint32_t* tile_order_next;  // delta pass: writes the next density pass's order (NULL: none)
+    const uint32_t* tile_order; // both passes: tile at each position (NULL: identity)
+    uint32_t* tile_order_next;  // delta pass: writes the next iteration's order here (NULL: none)
 };
 
 constexpr uint32_t kCostBuckets = 32;

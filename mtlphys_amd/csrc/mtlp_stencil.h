@@ -667,7 +667,10 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
         return;
     }
     const uint32_t nbt = gridDim.x - hb;
-    const uint32_t tile = xcd_block(blockIdx.x - hb, nbt);
+    const uint32_t pos = xcd_block(blockIdx.x - hb, nbt);
+    // the tiles in the density pass's order (heaviest first), else in place
+    const uint32_t tile =
+        nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
     if (nl.tile_order_next && threadIdx.x < 64u) order_tile(nl, tile, nbt);
     const uint32_t s = tile * kB + threadIdx.x;
     if (!delta_takes_slot(s, a.n, nl)) return;
