@@ -328,14 +328,10 @@ def _chunks_and_counts(orc, params, x, v, dt):
     return K, bh["nbr"]
 
 
-def test_dense_regions_chunk_paths(mtlp, orc):
-    """Blobs of three densities drive the density pass through every way it
-    keeps a slot's stencil chunks: all in LDS (<= 16), the global scratch
-    (17..32, with and without a neighbour list of <= 96 entries), and the
-    second stencil walk (> 32).  Lists on and off must agree bit for bit
-    over two steps (the second starts from marks the first left, if any),
-    and each iteration's kernels must match the FP64 oracle."""
-    n = 20000
+def _blob_case(mtlp, n=20000):
+    """A cube-drop lattice with three blobs of particles packed 1.2 h, 4 h
+    and 6 h wide: slots with few, many and very many stencil chunks, and
+    tiles of very different cost."""
     p = mtlp.scene_params(3, n)
     x, v = mtlp.generate_scene(3, n, 5, p)
     rng = np.random.default_rng(8)
@@ -345,11 +341,23 @@ def test_dense_regions_chunk_paths(mtlp, orc):
                               (3600, 7000, 6.0, (0.0, 0.0, 3.0))):
         x[lo:hi, :3] = (centre + np.array(off) * h + rng.uniform(-0.5, 0.5, (hi - lo, 3)) * side * h).astype(
             np.float32)
+    return p, x, v
+
+
+def test_dense_regions_chunk_paths(mtlp, orc):
+    """Blobs of three densities drive the density pass through every way it
+    keeps a slot's stencil chunks: all in LDS (<= 15), the global scratch
+    (16..32, with and without a neighbour list of <= 96 entries), and the
+    second stencil walk (> 32).  Lists on and off must agree bit for bit
+    over two steps (the second runs its tiles in the order the first left),
+    and each iteration's kernels must match the FP64 oracle."""
+    n = 20000
+    p, x, v = _blob_case(mtlp, n)
     dt = 1.0 / 120.0
     K, cnt = _chunks_and_counts(orc, p, x, v, dt)
-    assert (K <= 16).sum() > 1000
-    assert ((K > 16) & (K <= 32) & (cnt <= 96)).sum() > 50
-    assert ((K > 16) & (K <= 32) & (cnt > 96)).sum() > 50
+    assert (K <= 15).sum() > 1000
+    assert ((K > 15) & (K <= 32) & (cnt <= 96)).sum() > 50
+    assert ((K > 15) & (K <= 32) & (cnt > 96)).sum() > 50
     assert (K > 32).sum() > 50
     outs = {}
     for lists in (False, True):
@@ -365,6 +373,36 @@ def test_dense_regions_chunk_paths(mtlp, orc):
     for a, b in zip(outs[False], outs[True]):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
     _kernel_level(mtlp, orc, p, x, v, dt)
+
+
+@pytest.mark.parametrize("iterations", [4, 3])
+def test_tile_order_does_not_change_results(mtlp, iterations):
+    """The stencil passes take their tiles heaviest first, in an order the
+    previous iteration measured; with odd or even iteration counts, with the
+    stencil workgroup size changing between steps (which invalidates the
+    order) and with lists off (no order at all), four steps must give the
+    same bits."""
+    n = 20000
+    p, x, v = _blob_case(mtlp, n)
+    p.iterations = iterations
+    dt = 1.0 / 120.0
+    runs = {"lists off": (False, [64, 64, 64, 64]), "lists, 64": (True, [64, 64, 64, 64]),
+            "lists, sizes change": (True, [64, 128, 128, 64])}
+    outs = {}
+    for name, (lists, blocks) in runs.items():
+        with mtlp.Engine(0, p) as e:
+            e.reset(n, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+            e.set_neighbor_lists(lists)
+            outs[name] = []
+            for b in blocks:
+                e.set_stencil_block(b)
+                e.step(dt)
+                outs[name] += [e.positions_buffer().read(), e.lambdas_buffer().read()]
+            assert e.error_flags() == 0
+    for name in runs:
+        for a, b in zip(outs["lists off"], outs[name]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=name)
 
 
 def test_step_free_running_vs_fp64(mtlp, orc):
