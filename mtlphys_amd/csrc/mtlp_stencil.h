@@ -105,18 +105,31 @@ __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* 
 // Visits the stencil of slot s in walk order, one chunk of at most 32
 // consecutive candidate slots at a time: body(c0, mask), bit k of mask set
 // when slot c0 + k is a neighbour (neighbor_mask).
-template <class Body>
+// With kUnrollRows the 9-row loop is unrolled, so the rows' slot ranges
+// stay in registers: indexed by a loop counter they go to scratch memory
+// when LDS holds other data (the density pass's chunk columns), 18 extra
+// vector-memory reads per slot.
+template <bool kUnrollRows = false, class Body>
 __device__ __forceinline__ void for_each_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
                                                const float4* __restrict__ cur,
                                                const uint32_t* __restrict__ cell_start, uint32_t n,
                                                Body&& body) {
     const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
     const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
+    if constexpr (kUnrollRows) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const uint32_t b = rows.b[k];
+            for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
+                body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+        }
+    } else {
 #pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-        const uint32_t b = rows.b[k];
-        for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
-            body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+        for (int k = 0; k < 9; ++k) {
+            const uint32_t b = rows.b[k];
+            for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
+                body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+        }
     }
 }
 
@@ -178,7 +191,7 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
                                                   const NbrList& nl, uint2* le, uint32_t lst, uint32_t& nbrs) {
     uint32_t K = 0, c = 0;
-    for_each_chunk(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
+    for_each_chunk<true>(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
         const uint2 e = make_uint2(c0, mask);
         if (K < kLdsChunks) le[K * lst] = e;
         else if (K < kChunkScratch) nl.spill[(size_t)K * nl.stride + s] = e;
