@@ -222,8 +222,6 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->tile_plan, (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_order[0], (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_order[1], (na + 63) / 64));
-    HIP_TRY(dev_alloc(&e->tile_count, kTileCounters));
-    HIP_TRY(hipMemset(e->tile_count, 0, kTileCounters * sizeof(uint32_t)));
     e->tile_order_tiles[0] = e->tile_order_tiles[1] = 0;
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));

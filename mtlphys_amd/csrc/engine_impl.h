@@ -171,7 +171,6 @@ struct mtlp_engine {
     uint32_t* heavy = nullptr;        // 2 x kHeavyCap heavy-slot queues
     uint32_t* heavy_count = nullptr;  // their 2 counters
     uint32_t* tile_plan = nullptr;    // ceil(n_alloc / 64): density-pass tile costs (NbrList::tile_plan)
-    uint32_t* tile_count = nullptr;   // kTileCounters
     // two tile orders (ceil(n_alloc / 64) each): an iteration's density and
     // delta passes follow tile_order[tile_order_cur] while the delta pass
     // writes the other for the next iteration
@@ -212,7 +211,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(tile_plan); dev_free(tile_count); dev_free(tile_order[0]); dev_free(tile_order[1]);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]);
         tile_order_tiles[0] = tile_order_tiles[1] = 0;
         n_alloc = 0;
     }
@@ -241,20 +240,19 @@ struct mtlp_engine {
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;
-        b.tile_count = tile_count;
         return b;
     }
 
     uint32_t stencil_tiles() const { return (n + stencil_block - 1) / stencil_block; }
 
-    // lists of solver iteration `it` (its heavy queue and tile counters by
-    // parity); the passes follow tile_order[tile_order_cur] when it is a
-    // permutation of this launch's tiles
+    // lists of solver iteration `it` (its heavy queue by parity); the
+    // passes follow tile_order[tile_order_cur] when it is a permutation of
+    // this launch's tiles
     NbrList nbr_list(uint32_t it) const {
         if (!use_lists) return NbrList{};
         const uint32_t c = tile_order_cur;
         return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
-                       heavy_count + ((it + 1u) & 1u), tile_plan, tile_count, it & 1u,
+                       heavy_count + ((it + 1u) & 1u), tile_plan,
                        tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u]};
     }
     // after enqueueing an iteration's passes with lists: its delta pass

@@ -26,11 +26,10 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
     const float4* __restrict__ x, const float4* __restrict__ v, StepArgs a,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ tickets, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ tile_count) {
+    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) *big_count = 0u;  // the big-cell list of this build starts empty
     if (i < 2 && heavy_count) heavy_count[i] = 0u;  // so do the heavy-slot queues of its iterations
-    if (i < kTileCounters / 2 && tile_count) tile_count[i] = 0u;  // and the first density pass's tile counters
     if (i >= a.n) return;
     const float4 xi = x[i];
     const float4 vi = a.predict ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -227,7 +226,7 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 hipError_t launch_predict_hash_count(const HashBufs& b, const StepArgs& a, hipStream_t s) {
     const uint32_t grid = blocks_for(a.n > 0 ? a.n : 1, kBlock);
     hipLaunchKernelGGL(k_predict_hash_count, dim3(grid), dim3(kBlock), 0, s, b.x, b.v, a, b.keys,
-                       b.tickets, b.counts, b.big_count, b.heavy_count, b.tile_count);
+                       b.tickets, b.counts, b.big_count, b.heavy_count);
     return hipGetLastError();
 }
 

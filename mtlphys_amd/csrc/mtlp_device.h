@@ -171,6 +171,5 @@ __device__ __forceinline__ uint32_t xcd_of_position(uint32_t t, uint32_t nb) {
     const uint32_t big = r * (q + 1u);  // the first r ranges hold q + 1 positions
     return t < big ? t / (q + 1u) : r + (t - big) / q;  // q > 0 whenever t >= big
 }
-static_assert(kTileCounters == 2 * kNumXcd * kCostBuckets, "tile counters per XCD");
 
 }  // namespace mtlp

@@ -108,7 +108,6 @@ struct HashBufs {
     uint32_t* err;
     uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
     uint32_t* heavy_count; // 2 heavy-queue counters, cleared by the build; may be NULL
-    uint32_t* tile_count;  // tile-cost counters of the first iteration, cleared by the build; may be NULL
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------
@@ -173,20 +172,20 @@ struct NbrList {
     uint32_t* heavy_count;       // their number (may exceed kHeavyCap)
     uint32_t* heavy_count_next;  // the next iteration's counter
     // Tile order: each density pass times its tiles (one stencil workgroup
-    // of slots each) and ranks them by cost within their XCD's range; the
-    // delta pass that follows turns the ranks into the order in which the
-    // next iteration's passes take them, heaviest first, so that their last
-    // workgroups are short ones.  The results do not depend on the order.
-    uint32_t* tile_plan;        // per tile: cost bucket << 24 | rank in (XCD, bucket)
-    uint32_t* tile_count;       // 2 x kNumXcd x kCostBuckets counters, by iteration parity
-    uint32_t tile_parity;
+    // of slots each) into cost buckets; the delta pass that follows turns
+    // them into the order in which the next iteration's passes take the
+    // tiles of each XCD's range, heaviest buckets first and slot order
+    // inside a bucket, so that their last workgroups are short ones.  The
+    // results do not depend on the order.
+    uint32_t* tile_plan;        // per tile: its cost bucket
     const uint32_t* tile_order; // both passes: tile at each position (NULL: identity)
     uint32_t* tile_order_next;  // delta pass: writes the next iteration's order here (NULL: none)
 };
 
 constexpr uint32_t kCostBuckets = 32;
 constexpr uint32_t kCostTicks = 500;  // bucket width: 5 us of the 100 MHz real-time clock
-constexpr uint32_t kTileCounters = 2 * 8 * kCostBuckets;
+constexpr uint32_t kCostSpread = 8;   // a range whose heaviest tiles are at most this many buckets
+                                      // above its median keeps the slot order
 
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
 constexpr uint32_t kStencilBlocks[] = {64, 128, 256, 512};

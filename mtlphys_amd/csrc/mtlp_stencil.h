@@ -621,45 +621,63 @@ constexpr uint32_t heavy_blocks() {
 }
 
 // Times a stencil workgroup's tile (its first wave, from the start to the
-// exit of thread 0) and ranks the tile in its XCD's cost bucket, for the
-// order of the next density pass (NbrList::tile_order).
+// exit of thread 0) into a cost bucket, for the order of the next
+// iteration's passes (NbrList::tile_order).
 struct TileTimer {
     uint64_t t0;
     uint32_t* plan;
-    uint32_t* count;
-    uint32_t tile, nb;
-    __device__ TileTimer(const NbrList& nl, uint32_t tile_, uint32_t nb_)
-        : t0(__builtin_amdgcn_s_memrealtime()), plan(nl.tile_plan),
-          count(nl.tile_plan ? nl.tile_count + nl.tile_parity * (kTileCounters / 2) : nullptr), tile(tile_), nb(nb_) {}
+    uint32_t tile;
+    __device__ TileTimer(const NbrList& nl, uint32_t tile_)
+        : t0(__builtin_amdgcn_s_memrealtime()), plan(nl.tile_plan), tile(tile_) {}
     __device__ ~TileTimer() {
         if (plan && threadIdx.x == 0) {
             const uint64_t dt = __builtin_amdgcn_s_memrealtime() - t0;
-            const uint32_t b = (uint32_t)min(dt / kCostTicks, (uint64_t)(kCostBuckets - 1u));
-            const uint32_t rank = atomicAdd(count + xcd_of_position(tile, nb) * kCostBuckets + b, 1u);
-            plan[tile] = b << 24 | rank;
+            plan[tile] = (uint32_t)min(dt / kCostTicks, (uint64_t)(kCostBuckets - 1u));
         }
     }
 };
 
-// The delta pass's wave 0 of tile `tile` (of nb): the tile's position in
-// the next density pass's order, heavier cost buckets of its XCD's range
-// first; the workgroup of tile 0 clears the counters that pass will use.
-// Runs with the whole wave active.
-__device__ __forceinline__ void order_tile(const NbrList& nl, uint32_t tile, uint32_t nb) {
+// A delta launch's order workgroup for XCD range x of nb tiles (wave 0
+// works): the next iteration's order of the range from the cost buckets the
+// density pass left, heavier buckets first and slot order inside a bucket;
+// a range whose heaviest bucket is at most kCostSpread above its median
+// keeps the slot order.  Within one wave LDS operations complete in order,
+// so the histogram and the running bucket positions need no barrier.
+__device__ __forceinline__ void order_range(const NbrList& nl, uint32_t x, uint32_t nb) {
+    __shared__ uint32_t hist[kCostBuckets], next_pos[kCostBuckets];
+    if (threadIdx.x >= 64u) return;
     const uint32_t lane = __lane_id();
-    const uint32_t* cnt = nl.tile_count + nl.tile_parity * (kTileCounters / 2);
-    const uint32_t e = nl.tile_plan[tile];
-    const uint32_t b = e >> 24, rank = e & 0xffffffu;
-    const uint32_t x = xcd_of_position(tile, nb);
-    const uint32_t c = lane < kCostBuckets && lane > b ? cnt[x * kCostBuckets + lane] : 0u;
-    const uint32_t heavier = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_scan(c), 63);
-    if (lane == 0) {
-        const uint32_t p = xcd_range_start(x, nb) + heavier + rank;
-        if (p < nb) nl.tile_order_next[p] = tile;
+    const uint32_t lo = xcd_range_start(x, nb), hi = xcd_range_start(x + 1u, nb);
+    if (lane < kCostBuckets) hist[lane] = 0u;
+    for (uint32_t t = lo + lane; t < hi; t += 64u) atomicAdd(&hist[min(nl.tile_plan[t], kCostBuckets - 1u)], 1u);
+    const uint32_t cb = lane < kCostBuckets ? hist[lane] : 0u;
+    const uint32_t incl = wave_incl_scan(cb);  // tiles of the range in buckets <= lane
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    const uint64_t past_half = __ballot(lane < kCostBuckets && 2u * incl >= total);
+    const uint64_t nonempty = __ballot(cb != 0u);
+    const uint32_t med = past_half ? (uint32_t)__builtin_ctzll(past_half) : 0u;
+    const uint32_t top = nonempty ? 63u - (uint32_t)__builtin_clzll(nonempty) : 0u;
+    if (top <= med + kCostSpread) {
+        for (uint32_t t = lo + lane; t < hi; t += 64u) nl.tile_order_next[t] = t;
+        return;
     }
-    if (tile == 0)
-        for (uint32_t k = lane; k < kTileCounters / 2; k += 64u)
-            nl.tile_count[(nl.tile_parity ^ 1u) * (kTileCounters / 2) + k] = 0u;
+    if (lane < kCostBuckets) next_pos[lane] = lo + total - incl;  // first position of each bucket
+    const uint64_t below_me = (1ull << lane) - 1ull;
+    for (uint32_t c = lo; c < hi; c += 64u) {
+        const uint32_t t = c + lane;
+        const bool in = t < hi;
+        const uint32_t b = in ? min(nl.tile_plan[t], kCostBuckets - 1u) : 0u;
+        uint64_t same = __ballot(in);  // lanes of this chunk in my bucket
+        for (uint32_t k = 0; k < 5; ++k) {
+            const uint64_t m = __ballot(in && ((b >> k) & 1u));
+            same &= ((b >> k) & 1u) ? m : ~m;
+        }
+        if (in) {
+            const uint32_t p = next_pos[b] + (uint32_t)__popcll(same & below_me);
+            if (p < hi) nl.tile_order_next[p] = t;
+        }
+        if (in && (same >> lane) == 1ull) next_pos[b] += (uint32_t)__popcll(same);  // the group's last lane
+    }
 }
 
 template <uint32_t kB, bool kTape>
@@ -679,12 +697,17 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
         }
         return;
     }
-    const uint32_t nbt = gridDim.x - hb;
-    const uint32_t pos = xcd_block(blockIdx.x - hb, nbt);
+    // then, with an order to write, one workgroup per XCD range for it
+    const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
+    const uint32_t nbt = gridDim.x - hb - ob;
+    if (blockIdx.x < hb + ob) {
+        order_range(nl, blockIdx.x - hb, nbt);
+        return;
+    }
+    const uint32_t pos = xcd_block(blockIdx.x - hb - ob, nbt);
     // the tiles in the density pass's order (heaviest first), else in place
     const uint32_t tile =
         nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
-    if (nl.tile_order_next && threadIdx.x < 64u) order_tile(nl, tile, nbt);
     const uint32_t s = tile * kB + threadIdx.x;
     if (!delta_takes_slot(s, a.n, nl)) return;
     with_scene(s, a, [&](const SceneDev& sc) {

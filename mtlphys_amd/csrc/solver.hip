@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     // pass left (heaviest first within each XCD's range), else in place
     const uint32_t tile =
         kList && nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nb - 1u) : pos;
-    const TileTimer timer(kList ? nl : NbrList{}, tile, nb);
+    const TileTimer timer(kList ? nl : NbrList{}, tile);
     const uint32_t s0 = tile * kB + threadIdx.x;
     const uint32_t n = a.n;
     // The wave's slots are handed to its lanes in order of their current
@@ -167,7 +167,8 @@ template <uint32_t kB>
 void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
                    hipStream_t s) {
     // with lists, the heavy queue's workgroups in front
-    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() : 0u)), block(kB);
+    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() : 0u) + (nl.tile_order_next ? kNumXcd : 0u)),
+        block(kB);
     if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
     else hipLaunchKernelGGL((k_apply_delta<kB, false>), grid, block, 0, s, cur, cs, a, next, nl);
 }
