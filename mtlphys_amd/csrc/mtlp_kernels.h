@@ -147,8 +147,9 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // that (none in practice) gives kCountStencil, and readers walk the stencil.
 constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
-// 15 rather than 16: with 7,680 bytes of LDS per one-wave workgroup 21
-// workgroups run on a CU at once, with 8,192 bytes 18 (measured).
+// 15 rather than 16: with 7,680 bytes of LDS per one-wave workgroup the LDS
+// lets 21 workgroups share a CU (the kernel's VGPRs then allow 20), with
+// 8,192 bytes 18 (measured).
 constexpr uint32_t kLdsChunks = 15;
 constexpr uint32_t kChunkScratch = 32;
 //

@@ -481,8 +481,8 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
 }
 
 // Work item s of the delta pass: its slot unless the slot is queued as
-// heavy (then the heavy launch takes it).  Item 0 clears the counter of the
-// next iteration's queue.
+// heavy (then one of the launch's heavy waves takes it).  Item 0 clears the
+// counter of the next iteration's queue.
 __device__ __forceinline__ bool delta_takes_slot(uint32_t s, uint32_t n, const NbrList& nl) {
     if (s == 0) *nl.heavy_count_next = 0u;
     return s < n && (nl.count[s] & kCountQueued) == 0u;
