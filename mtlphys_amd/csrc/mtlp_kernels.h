@@ -163,7 +163,7 @@ constexpr uint32_t kChunkScratch = 32;
 constexpr uint32_t kCountChunks = 0x80000000u;   // | K: chunks in spill
 constexpr uint32_t kCountQueued = 0x40000000u;   // in the heavy queue
 constexpr uint32_t kCountStencil = kCountChunks | 0xffffu;
-constexpr uint32_t kHeavyCap = 4096;
+constexpr uint32_t kHeavyCap = 16384;
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride

@@ -613,7 +613,7 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
 // come first in dispatch order and run the heavy queue, a wave per queued
 // slot (grid-stride over the queue, kHeavyWaves waves); the others take
 // the slots.  Returns the workgroups for n slots.
-constexpr uint32_t kHeavyWaves = 512;
+constexpr uint32_t kHeavyWaves = 2048;
 
 template <uint32_t kB>
 constexpr uint32_t heavy_blocks() {
