@@ -190,6 +190,32 @@ def adjoint_leg(torch, reps=3):
             "finite": finite, "error_flags": flags}
 
 
+def step_curve(steps, torch, window=5):
+    """ms per step of the 1M dam break in windows of `window` steps from the
+    initial state (the step slows as the column collapses into dense
+    corners), and the slots with more than 96 neighbours at the end."""
+    import mtlphys_amd as mtlp
+    params = mtlp.scene_params(SCENE_DAM, N_PARTICLES)
+    x, v = mtlp.generate_scene(SCENE_DAM, N_PARTICLES, 1, params)
+    eng = mtlp.Engine(0, params)
+    eng.reset(N_PARTICLES, mtlp.capi.SCENE_EMPTY)
+    eng.upload_state(x, v)
+    eng.set_stencil_block(64)
+    stream = torch.cuda.current_stream()
+    out = []
+    for _ in range(max(1, steps // window)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(window):
+            eng.step(DT, stream)
+        torch.cuda.synchronize()
+        out.append(round((time.perf_counter() - t0) / window * 1e3, 3))
+    counts = eng.neighbor_counts_buffer().read()
+    eng.close()
+    return {"workload": "dam_break_1m", "window": window, "ms_per_step": out,
+            "slots_over_96_neighbours": int((counts > 96).sum()), "max_neighbours": int(counts.max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +226,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the forward+adjoint leg and print its JSON (for profiling)")
+    ap.add_argument("--step-curve", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS steps of the 1M dam break from its initial state in windows of "
+                         "5 (one-wave stencil workgroups) and print ms per step per window")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (reference kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -212,6 +241,9 @@ def main():
     import torch
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(torch, reps=2)), flush=True)
+        return
+    if args.step_curve:
+        print(json.dumps(step_curve(args.step_curve, torch)), flush=True)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
