@@ -375,6 +375,40 @@ def test_dense_regions_chunk_paths(mtlp, orc):
     _kernel_level(mtlp, orc, p, x, v, dt)
 
 
+def test_heavy_queue_overflow(mtlp, orc):
+    """A dense cube (about 150 neighbours per interior particle) gives more
+    slots with over 96 neighbours or over 32 stencil chunks than the delta
+    pass's heavy queue holds (16,384): the first ones get a wave each, the
+    rest are walked by their own lane.  Lists on and off must agree bit for
+    bit over two steps, and each iteration's kernels must match the FP64
+    oracle."""
+    n, m = 40000, 30000
+    p = mtlp.scene_params(3, n)
+    x, v = mtlp.generate_scene(3, n, 5, p)
+    rng = np.random.default_rng(11)
+    centre = x[:, :3].mean(axis=0)
+    side = (m / 36.0) ** (1.0 / 3.0)  # 36 particles per h^3
+    x[:m, :3] = (centre + rng.uniform(-0.5, 0.5, (m, 3)) * side * p.h).astype(np.float32)
+    dt = 1.0 / 120.0
+    K, cnt = _chunks_and_counts(orc, p, x, v, dt)
+    heavy = (K > 32) | (cnt > 96)
+    assert heavy.sum() > 16384 + 2000, heavy.sum()
+    outs = {}
+    for lists in (False, True):
+        with mtlp.Engine(0, p) as e:
+            e.reset(n, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+            e.set_neighbor_lists(lists)
+            outs[lists] = []
+            for _ in range(2):
+                e.step(dt)
+                outs[lists] += [e.positions_buffer().read(), e.lambdas_buffer().read()]
+            assert e.error_flags() == 0
+    for a, b in zip(outs[False], outs[True]):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    _kernel_level(mtlp, orc, p, x, v, dt)
+
+
 @pytest.mark.parametrize("iterations", [4, 3])
 def test_tile_order_does_not_change_results(mtlp, iterations):
     """The stencil passes take their tiles heaviest first, in an order the
