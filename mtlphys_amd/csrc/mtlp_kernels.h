@@ -73,7 +73,6 @@ struct SceneDev {
     uint32_t cell_off;    // first global cell key of the scene
 };
 
-// Step-wide inputs shared by every scene.
 // Kinematic sphere of the sphere-drop scene (SphereParams, Shared.h:78-81):
 // solver iterates inside it are pushed to its surface.
 struct SphereDev {
@@ -81,6 +80,7 @@ struct SphereDev {
     uint32_t on;
 };
 
+// Step-wide inputs shared by every scene.
 struct StepArgs {
     const SceneDev* scenes;
     const uint32_t* p_off;  // n_scenes + 1 particle offsets
@@ -200,11 +200,10 @@ hipError_t launch_density_lambda(float4* cur, const uint32_t* cell_start,
                                  uint32_t block, hipStream_t s);
 hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, const StepArgs& a,
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
-// Scatter back + finalize; with foam non-NULL also computeFoam from the
-// step's neighbour counts (nbr, original order).
 // Scatter back + finalize in original order: particle i takes its result
-// from slot slot_of[i]; with lam_sorted (the last density iterate, lambda in
-// .w) it also writes lambda_orig.
+// from slot slot_of[i]; with foam non-NULL also computeFoam from the step's
+// neighbour counts (nbr, original order); with lam_sorted (the last density
+// iterate, lambda in .w) it also writes lambda_orig.
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
                            const uint32_t* nbr, float* foam, const float4* lam_sorted, float* lambda_orig,
