@@ -1,6 +1,7 @@
 """Benchmark of the PBF step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batched] [--no-adjoint]
+                    [--no-late]
 
 Workload: the 1M-particle dam break (h = 0.02, box 2.027 x 2.027 x 1.0135,
 102 x 102 x 51 = 530,604 cells, 4 solver iterations, dt = 1/120), seeded
@@ -17,6 +18,8 @@ One JSON line on stdout:
                rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/), else null
   cpu_baseline the oracle's FP32 step on this host's cores (OpenMP), timed
                on a bounded sample of the same workload
+  late_steps   ms per step in windows of 20 over the first 120 steps of the
+               same dam break (the step slows as dense corners form)
 """
 import argparse
 import json
@@ -224,6 +227,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
+    ap.add_argument("--no-late", action="store_true",
+                    help="skip the late-steps leg (ms per step over 120 steps from the initial state)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only run the forward+adjoint leg and print its JSON (for profiling)")
     ap.add_argument("--step-curve", type=int, default=0, metavar="STEPS",
@@ -343,6 +348,10 @@ def main():
         "kernels": kernels,
         "batched": None if args.no_batched else batched_leg(args.steps, args.warmup, stream, torch),
         "adjoint": None if args.no_adjoint else adjoint_leg(torch),
+        # the same dam break run on to step 120: the step slows as the column
+        # piles into dense corners (per-lane imbalance), which the timed window
+        # above is before
+        "late_steps": None if args.no_late else step_curve(120, torch, window=20),
         "cpu_baseline": None if args.no_cpu else cpu_baseline(x, v, params),
     }
     print(json.dumps(out), flush=True)
