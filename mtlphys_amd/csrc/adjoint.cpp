@@ -101,6 +101,7 @@ StepTape tape_of(const mtlp_engine* e, uint32_t slot) {
     t.spill = e->nspill;
     t.heavy = e->heavy;
     t.heavy_count = e->heavy_count;
+    t.lanes = e->lanes;
     t.stride = st;
     return t;
 }

@@ -217,8 +217,9 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->ncount, na));
     HIP_TRY(dev_alloc(&e->nspill, (size_t)kChunkScratch * na));
     HIP_TRY(dev_alloc(&e->heavy, 2 * (size_t)kHeavyCap));
-    HIP_TRY(dev_alloc(&e->heavy_count, 2));
-    HIP_TRY(hipMemset(e->heavy_count, 0, 2 * sizeof(uint32_t)));
+    HIP_TRY(dev_alloc(&e->heavy_count, 4));
+    HIP_TRY(hipMemset(e->heavy_count, 0, 4 * sizeof(uint32_t)));
+    HIP_TRY(dev_alloc(&e->lanes, 2 * (size_t)lanes_cap_for(na)));
     HIP_TRY(dev_alloc(&e->tile_plan, (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_order[0], (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_order[1], (na + 63) / 64));

@@ -169,7 +169,8 @@ struct mtlp_engine {
     uint32_t* ncount = nullptr;
     uint2* nspill = nullptr;  // kChunkScratch * n_alloc: density-pass chunk scratch
     uint32_t* heavy = nullptr;        // 2 x kHeavyCap heavy-slot queues
-    uint32_t* heavy_count = nullptr;  // their 2 counters
+    uint32_t* heavy_count = nullptr;  // their 2 counters, then the heavy-lane queues' 2
+    uint32_t* lanes = nullptr;        // 2 x lanes_cap_for(n_alloc) heavy-lane queues
     uint32_t* tile_plan = nullptr;    // ceil(n_alloc / 64): density-pass tile costs (NbrList::tile_plan)
     // two tile orders (ceil(n_alloc / 64) each): an iteration's density and
     // delta passes follow tile_order[tile_order_cur] while the delta pass
@@ -211,7 +212,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]);
         tile_order_tiles[0] = tile_order_tiles[1] = 0;
         n_alloc = 0;
     }
@@ -251,8 +252,10 @@ struct mtlp_engine {
     NbrList nbr_list(uint32_t it) const {
         if (!use_lists) return NbrList{};
         const uint32_t c = tile_order_cur;
+        const uint32_t lcap = lanes_cap_for(n_alloc);
         return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
-                       heavy_count + ((it + 1u) & 1u), tile_plan,
+                       heavy_count + ((it + 1u) & 1u), lanes + (size_t)(it & 1u) * lcap,
+                       heavy_count + 2u + (it & 1u), heavy_count + 2u + ((it + 1u) & 1u), lcap, tile_plan,
                        tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u]};
     }
     // after enqueueing an iteration's passes with lists: its delta pass

@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
     uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i == 0) *big_count = 0u;  // the big-cell list of this build starts empty
-    if (i < 2 && heavy_count) heavy_count[i] = 0u;  // so do the heavy-slot queues of its iterations
+    if (i < 4 && heavy_count) heavy_count[i] = 0u;  // so do the heavy-slot queues of its iterations
     if (i >= a.n) return;
     const float4 xi = x[i];
     const float4 vi = a.predict ? v[i] : make_float4(0.f, 0.f, 0.f, 0.f);

@@ -107,7 +107,7 @@ struct HashBufs {
     uint32_t* big_count;
     uint32_t* err;
     uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
-    uint32_t* heavy_count; // 2 heavy-queue counters, cleared by the build; may be NULL
+    uint32_t* heavy_count; // 4 heavy-queue counters, cleared by the build; may be NULL
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------
@@ -154,16 +154,25 @@ constexpr uint32_t kLdsChunks = 15;
 constexpr uint32_t kChunkScratch = 32;
 //
 // Such slots ("heavy": their walk is several times the typical one) also
-// go to a queue of up to kHeavyCap slots, marked kCountQueued in the count;
-// the delta pass's first workgroups give each queued slot a whole wave (one
-// lane alone would run long after the rest of the pass) and the others skip
-// them.
-// The queue is double-buffered by iteration parity: the delta pass of one
-// iteration clears the counter the next density pass appends to.
+// go to a queue, marked kCountQueued in the count, and the delta pass's
+// ordinary workgroups skip them (one heavy lane in an ordinary wave would
+// keep it running long after the rest of the pass).  Slots left to the
+// stencil (more than kChunkScratch chunks) go to a queue of up to kHeavyCap
+// whose slots get a whole wave each, 64 candidates at a time; slots whose
+// chunks are in the scratch go to a queue of up to lanes_cap whose slots the
+// delta pass's heavy-lane workgroups walk one per lane, so that those waves
+// hold heavy slots only (a short queue goes to the heavy waves instead).  A slot that finds its queue full stays with its
+// ordinary wave.  The queues are double-buffered by iteration parity: the
+// delta pass of one iteration clears the counters the next density pass
+// appends to.
 constexpr uint32_t kCountChunks = 0x80000000u;   // | K: chunks in spill
 constexpr uint32_t kCountQueued = 0x40000000u;   // in the heavy queue
 constexpr uint32_t kCountStencil = kCountChunks | 0xffffu;
 constexpr uint32_t kHeavyCap = 16384;
+// Heavy-lane queue entries per iteration parity for n slots: n / 8, a
+// multiple of 4,096 (so its workgroups are a multiple of 8 at every stencil
+// workgroup size and keep the XCD mapping of the workgroups after them).
+inline uint32_t lanes_cap_for(size_t n) { return (uint32_t)(((n >> 3) + 4095u) / 4096u * 4096u); }
 struct NbrList {
     uint4* blk;       // kListBlocks * stride
     uint32_t* count;  // stride
@@ -172,6 +181,10 @@ struct NbrList {
     uint32_t* heavy;             // kHeavyCap queued slots of this iteration
     uint32_t* heavy_count;       // their number (may exceed kHeavyCap)
     uint32_t* heavy_count_next;  // the next iteration's counter
+    uint32_t* lanes;             // lanes_cap queued heavy-lane slots of this iteration
+    uint32_t* lanes_count;       // their number (may exceed lanes_cap)
+    uint32_t* lanes_count_next;  // the next iteration's counter
+    uint32_t lanes_cap;          // lanes_cap_for(stride)
     // Tile order: each density pass times its tiles (one stencil workgroup
     // of slots each) into cost buckets; the delta pass that follows turns
     // them into the order in which the next iteration's passes take the
@@ -231,14 +244,17 @@ struct StepTape {
     uint32_t* lcnt;          // I count arrays
     uint2* spill;            // the density pass's chunk scratch (not part of the tape)
     uint32_t* heavy;         // 2 x kHeavyCap heavy-slot queues (iteration parity)
-    uint32_t* heavy_count;   // 2 counters
+    uint32_t* heavy_count;   // 4 counters: heavy queue, heavy-lane queue, by parity
+    uint32_t* lanes;         // 2 x lanes_cap_for(stride) heavy-lane queues
     size_t stride;
     float4* iterate(uint32_t k) const { return iter + k * stride; }
     float4* grad_sums(uint32_t k) const { return grad + k * stride; }
     float4* delta(uint32_t k) const { return dlt + k * stride; }
     NbrList list(uint32_t k) const {
         return NbrList{lblk + (size_t)k * kListBlocks * stride, lcnt + k * stride, (uint32_t)stride, spill,
-                       heavy + (k & 1u) * kHeavyCap, heavy_count + (k & 1u), heavy_count + ((k + 1u) & 1u)};
+                       heavy + (k & 1u) * kHeavyCap, heavy_count + (k & 1u), heavy_count + ((k + 1u) & 1u),
+                       lanes + (size_t)(k & 1u) * lanes_cap_for(stride), heavy_count + 2u + (k & 1u),
+                       heavy_count + 2u + ((k + 1u) & 1u), lanes_cap_for(stride)};
     }
 };
 

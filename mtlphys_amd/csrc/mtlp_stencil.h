@@ -354,14 +354,20 @@ __device__ __forceinline__ float lambda_of(double wsum, float gx, float gy, floa
 }
 
 // The list count code of a slot whose walk had K chunks and cnt neighbours
-// (see NbrList); a heavy slot also goes to the queue.
+// (see NbrList); a heavy slot also goes to its queue.
 __device__ __forceinline__ uint32_t list_code(uint32_t s, uint32_t K, uint32_t cnt, const NbrList& nl) {
     uint32_t code = kCountStencil;
     if (K <= kChunkScratch) code = cnt <= kMaxList ? cnt : kCountChunks | K;
-    if (code > kMaxList) {  // heavy: queued for the delta pass's wave-per-slot launch
+    if (code == kCountStencil) {  // a wave of the delta pass's heavy waves
         const uint32_t q = atomicAdd(nl.heavy_count, 1u);
         if (q < kHeavyCap) {
             nl.heavy[q] = s;
+            code |= kCountQueued;
+        }
+    } else if (code > kMaxList) {  // a lane of its heavy-lane workgroups
+        const uint32_t q = atomicAdd(nl.lanes_count, 1u);
+        if (q < nl.lanes_cap) {
+            nl.lanes[q] = s;
             code |= kCountQueued;
         }
     }
@@ -481,10 +487,13 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
 }
 
 // Work item s of the delta pass: its slot unless the slot is queued as
-// heavy (then one of the launch's heavy waves takes it).  Item 0 clears the
-// counter of the next iteration's queue.
+// heavy (then a heavy wave or a heavy-lane workgroup of the launch takes
+// it).  Item 0 clears the counters of the next iteration's queues.
 __device__ __forceinline__ bool delta_takes_slot(uint32_t s, uint32_t n, const NbrList& nl) {
-    if (s == 0) *nl.heavy_count_next = 0u;
+    if (s == 0) {
+        *nl.heavy_count_next = 0u;
+        *nl.lanes_count_next = 0u;
+    }
     return s < n && (nl.count[s] & kCountQueued) == 0u;
 }
 
@@ -611,9 +620,14 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
 
 // A delta launch over n slots with lists: workgroups [0, heavy_blocks<kB>)
 // come first in dispatch order and run the heavy queue, a wave per queued
-// slot (grid-stride over the queue, kHeavyWaves waves); the others take
-// the slots.  Returns the workgroups for n slots.
+// slot (grid-stride over the queue, kHeavyWaves waves); the next lanes_cap
+// / kB walk the heavy-lane queue, a slot per lane (those past its count
+// exit at once), unless it holds at most kLanesByWaves slots (then the heavy
+// waves take them after their own); then, with an order to write, one
+// workgroup per XCD range for it; the others take the slots.
 constexpr uint32_t kHeavyWaves = 2048;
+// heavy-lane queues up to this long are run by the heavy waves (8 slots each)
+constexpr uint32_t kLanesByWaves = 8 * kHeavyWaves;
 
 template <uint32_t kB>
 constexpr uint32_t heavy_blocks() {
@@ -686,30 +700,44 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
                                                   float4* __restrict__ next, float4* __restrict__ rr,
                                                   const NbrList& nl) {
     constexpr uint32_t hb = heavy_blocks<kB>();
+    // a short heavy-lane queue goes to the heavy waves too: one lane alone
+    // walks a heavy slot at a memory latency per four neighbours, which
+    // outlasts the pass unless many such lanes share it
+    const uint32_t l_end = min(*nl.lanes_count, nl.lanes_cap);
+    const bool lanes_by_waves = l_end <= kLanesByWaves;
     if (blockIdx.x < hb) {
         const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kB / 64u) + threadIdx.x / 64u));
         const uint32_t h_end = min(*nl.heavy_count, kHeavyCap);
-        for (uint32_t h = wave; h < h_end; h += kHeavyWaves) {
-            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)nl.heavy[h]);
+        const uint32_t end = h_end + (lanes_by_waves ? l_end : 0u);
+        for (uint32_t h = wave; h < end; h += kHeavyWaves) {
+            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)(h < h_end ? nl.heavy[h] : nl.lanes[h - h_end]));
             with_scene(s, a, [&](const SceneDev& sc) {
                 apply_delta_heavy<kTape>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
             });
         }
         return;
     }
-    // then, with an order to write, one workgroup per XCD range for it
-    const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
-    const uint32_t nbt = gridDim.x - hb - ob;
-    if (blockIdx.x < hb + ob) {
-        order_range(nl, blockIdx.x - hb, nbt);
-        return;
+    const uint32_t lb = nl.lanes_cap / kB;
+    uint32_t s;
+    if (blockIdx.x < hb + lb) {
+        const uint32_t h = (blockIdx.x - hb) * kB + threadIdx.x;
+        if (lanes_by_waves || h >= l_end) return;
+        s = nl.lanes[h];
+    } else {
+        const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
+        const uint32_t nbt = gridDim.x - hb - lb - ob;
+        if (blockIdx.x < hb + lb + ob) {
+            order_range(nl, blockIdx.x - hb - lb, nbt);
+            return;
+        }
+        const uint32_t pos = xcd_block(blockIdx.x - hb - lb - ob, nbt);
+        // the tiles in the density pass's order (heaviest first), else in place
+        const uint32_t tile =
+            nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
+        s = tile * kB + threadIdx.x;
+        if (!delta_takes_slot(s, a.n, nl)) return;
     }
-    const uint32_t pos = xcd_block(blockIdx.x - hb - ob, nbt);
-    // the tiles in the density pass's order (heaviest first), else in place
-    const uint32_t tile =
-        nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
-    const uint32_t s = tile * kB + threadIdx.x;
-    if (!delta_takes_slot(s, a.n, nl)) return;
+    // one copy of the per-slot walk for heavy-lane and ordinary workgroups
     with_scene(s, a, [&](const SceneDev& sc) {
         apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
     });

@@ -375,14 +375,18 @@ def test_dense_regions_chunk_paths(mtlp, orc):
     _kernel_level(mtlp, orc, p, x, v, dt)
 
 
-def test_heavy_queue_overflow(mtlp, orc):
-    """A dense cube (about 150 neighbours per interior particle) gives more
-    slots with over 96 neighbours or over 32 stencil chunks than the delta
-    pass's heavy queue holds (16,384): the first ones get a wave each, the
-    rest are walked by their own lane.  Lists on and off must agree bit for
-    bit over two steps, and each iteration's kernels must match the FP64
-    oracle."""
-    n, m = 40000, 30000
+@pytest.mark.parametrize("n,m", [(40000, 30000), (200000, 100000)])
+def test_heavy_queue_overflow(mtlp, orc, n, m):
+    """A dense cube of m particles (about 150 neighbours per interior one)
+    gives thousands of slots with over 32 stencil chunks (a wave each from
+    the delta pass's heavy queue) and more slots with over 96 neighbours in
+    at most 32 chunks than its heavy-lane queue holds (n / 8, rounded up to
+    4,096): at n = 40,000 that queue (8,192) is short enough for the heavy
+    waves to take its slots; at n = 200,000 (28,672) its slots are walked
+    one per lane in the heavy-lane workgroups, and both queues overflow into
+    the ordinary waves.
+    Lists on and off must agree bit for bit over two steps, and each
+    iteration's kernels must match the FP64 oracle."""
     p = mtlp.scene_params(3, n)
     x, v = mtlp.generate_scene(3, n, 5, p)
     rng = np.random.default_rng(11)
@@ -391,8 +395,9 @@ def test_heavy_queue_overflow(mtlp, orc):
     x[:m, :3] = (centre + rng.uniform(-0.5, 0.5, (m, 3)) * side * p.h).astype(np.float32)
     dt = 1.0 / 120.0
     K, cnt = _chunks_and_counts(orc, p, x, v, dt)
-    heavy = (K > 32) | (cnt > 96)
-    assert heavy.sum() > 16384 + 2000, heavy.sum()
+    lanes_cap = (n // 8 + 4095) // 4096 * 4096
+    assert (K > 32).sum() > 2000, (K > 32).sum()
+    assert ((K <= 32) & (cnt > 96)).sum() > lanes_cap + 2000, ((K <= 32) & (cnt > 96)).sum()
     outs = {}
     for lists in (False, True):
         with mtlp.Engine(0, p) as e:
