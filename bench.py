@@ -196,7 +196,8 @@ def adjoint_leg(torch, reps=3):
 def step_curve(steps, torch, window=5):
     """ms per step of the 1M dam break in windows of `window` steps from the
     initial state (the step slows as the column collapses into dense
-    corners), and the slots with more than 96 neighbours at the end."""
+    corners), the slots with more than 96 neighbours at the end, and the
+    per-stage times (HIP events) of one more step."""
     import mtlphys_amd as mtlp
     params = mtlp.scene_params(SCENE_DAM, N_PARTICLES)
     x, v = mtlp.generate_scene(SCENE_DAM, N_PARTICLES, 1, params)
@@ -214,9 +215,17 @@ def step_curve(steps, torch, window=5):
         torch.cuda.synchronize()
         out.append(round((time.perf_counter() - t0) / window * 1e3, 3))
     counts = eng.neighbor_counts_buffer().read()
+    # one more step with the engine's per-kernel events: us per stage
+    eng.set_profiling(True)
+    eng.step(DT, stream)
+    stages = {}
+    for name, ms in eng.stage_times():
+        stages[name] = round(stages.get(name, 0.0) + ms * 1e3, 1)
+    eng.set_profiling(False)
     eng.close()
     return {"workload": "dam_break_1m", "window": window, "ms_per_step": out,
-            "slots_over_96_neighbours": int((counts > 96).sum()), "max_neighbours": int(counts.max())}
+            "slots_over_96_neighbours": int((counts > 96).sum()), "max_neighbours": int(counts.max()),
+            "next_step_stage_us": stages}
 
 
 def main():
