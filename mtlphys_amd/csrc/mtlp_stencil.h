@@ -209,13 +209,24 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
 // takes four neighbours, with their loads in flight before any term runs,
 // and with `listing` stores them as block t of slot s's list (padded with
 // s), so the lanes of a wave store block t together.
-template <class Chunk, class Load, class Term>
+// With kAhead the next chunk is fetched when this one starts, so a walk
+// through the global scratch waits one memory latency per chunk less (the
+// delta pass's heavy slots; the density pass, whose first chunks are in LDS,
+// is faster without).
+template <bool kAhead = false, class Chunk, class Load, class Term>
 __device__ __forceinline__ void visit_chunks(Chunk&& chunk, uint32_t K, uint32_t s, const NbrList& nl,
                                              bool listing, Load&& load, Term&& term) {
     uint32_t k = 0, b = 0, m = 0;
+    uint2 ahead = kAhead && K ? chunk(0u) : make_uint2(0u, 0u);
     auto next = [&](uint32_t& j) {
         while (m == 0u && k < K) {
-            const uint2 e = chunk(k);
+            uint2 e;
+            if (kAhead) {
+                e = ahead;
+                if (k + 1u < K) ahead = chunk(k + 1u);
+            } else {
+                e = chunk(k);
+            }
             ++k;
             b = e.x;
             m = e.y;
@@ -283,8 +294,8 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
         }
         const uint32_t K = cnt & 0xffffu;
         if (kScratch && (cnt & kCountChunks) && K != 0xffffu) {
-            visit_chunks([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, K, s, nl, false, load,
-                         term);
+            visit_chunks<true>([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, K, s, nl, false,
+                               load, term);
             return;
         }
     }
