@@ -209,24 +209,19 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
 // takes four neighbours, with their loads in flight before any term runs,
 // and with `listing` stores them as block t of slot s's list (padded with
 // s), so the lanes of a wave store block t together.
-// With kAhead the next chunk is fetched when this one starts, so a walk
-// through the global scratch waits one memory latency per chunk less (the
-// delta pass's heavy slots; the density pass, whose first chunks are in LDS,
-// is faster without).
-template <bool kAhead = false, class Chunk, class Load, class Term>
+// From chunk kAheadFrom on, the next chunk is fetched when this one starts,
+// so a walk through the global scratch waits one memory latency per chunk
+// less (the delta pass's heavy slots from the first chunk; the density pass
+// once its walk leaves the chunks kept in LDS, which need no prefetch).
+template <uint32_t kAheadFrom = 0xffffffffu, class Chunk, class Load, class Term>
 __device__ __forceinline__ void visit_chunks(Chunk&& chunk, uint32_t K, uint32_t s, const NbrList& nl,
                                              bool listing, Load&& load, Term&& term) {
     uint32_t k = 0, b = 0, m = 0;
-    uint2 ahead = kAhead && K ? chunk(0u) : make_uint2(0u, 0u);
+    uint2 ahead = kAheadFrom == 0u && K ? chunk(0u) : make_uint2(0u, 0u);
     auto next = [&](uint32_t& j) {
         while (m == 0u && k < K) {
-            uint2 e;
-            if (kAhead) {
-                e = ahead;
-                if (k + 1u < K) ahead = chunk(k + 1u);
-            } else {
-                e = chunk(k);
-            }
+            const uint2 e = k >= kAheadFrom ? ahead : chunk(k);
+            if (k + 1u >= kAheadFrom && k + 1u < K) ahead = chunk(k + 1u);
             ++k;
             b = e.x;
             m = e.y;
@@ -294,8 +289,8 @@ __device__ __forceinline__ void for_each_neighbor_of(uint32_t s, const float4& p
         }
         const uint32_t K = cnt & 0xffffu;
         if (kScratch && (cnt & kCountChunks) && K != 0xffffu) {
-            visit_chunks<true>([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, K, s, nl, false,
-                               load, term);
+            visit_chunks<0u>([&](uint32_t k) { return nl.spill[(size_t)k * nl.stride + s]; }, K, s, nl, false,
+                             load, term);
             return;
         }
     }
@@ -464,7 +459,7 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
         const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, nl, le, lst, cnt);
         auto load = [&](uint32_t j) { return cur[j]; };
         auto term = [&](uint32_t, const float4& q) { density_term(pi, q, sd, wsum, gx, gy, gz, gsq); };
-        visit_chunks(
+        visit_chunks<kLdsChunks>(
             [&](uint32_t k) { return k < kLdsChunks ? le[k * lst] : nl.spill[(size_t)k * nl.stride + s]; },
             min(K, kChunkScratch), s, nl, K <= kChunkScratch, load, term);
         if (K > kChunkScratch)
