@@ -716,7 +716,8 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
         const uint32_t h_end = min(*nl.heavy_count, kHeavyCap);
         const uint32_t end = h_end + (lanes_by_waves ? l_end : 0u);
         for (uint32_t h = wave; h < end; h += kHeavyWaves) {
-            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)(h < h_end ? nl.heavy[h] : nl.lanes[h - h_end]));
+            const uint32_t q = h < h_end ? nl.heavy[h] : nl.lanes[h - h_end];  // h is wave-uniform
+            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
             with_scene(s, a, [&](const SceneDev& sc) {
                 apply_delta_heavy<kTape>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
             });
