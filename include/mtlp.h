@@ -186,6 +186,18 @@ mtlp_status mtlp_build_spatial_hash(mtlp_engine* e, mtlp_stream stream);
 size_t mtlp_scan_scratch_bytes(uint32_t n);
 mtlp_status mtlp_exclusive_scan(mtlp_engine* e, mtlp_stream stream, const uint32_t* d_in,
                                 uint32_t* d_out, void* d_scratch, uint32_t n);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mtlp_exclusive_scan without d_scratch and without an
+ * engine.  Its working memory is a fixed array in the library's device
+ * memory (512 KiB per GPU the library runs on, held until the process
+ * ends), cleared by a kernel at the start of every call; the call allocates
+ * nothing.  n <= MTLP_NOALLOC_SCAN_MAX, else MTLP_E_RANGE; d_in and d_out as
+ * for mtlp_exclusive_scan.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (on different streams, or from several threads)
+ * race and give wrong results; calls in one stream's order do not. */
+#define MTLP_NOALLOC_SCAN_MAX 0x10000000u
+mtlp_status mtlp_exclusive_scan_noalloc(mtlp_stream stream, const uint32_t* d_in, uint32_t* d_out, uint32_t n);
 
 /* Accessors (Engine.hpp:78-90). */
 mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, size_t* bytes);

@@ -100,6 +100,7 @@ SIGNATURES = {
     "mtlp_build_spatial_hash": (_I, [_P, _P]),
     "mtlp_scan_scratch_bytes": (ctypes.c_size_t, [_U32]),
     "mtlp_exclusive_scan": (_I, [_P, _P, _P, _P, _P, _U32]),
+    "mtlp_exclusive_scan_noalloc": (_I, [_P, _P, _P, _U32]),
     "mtlp_get_buffer": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
     "mtlp_read_buffer": (_I, [_P, _I, _P, ctypes.c_size_t]),
     "mtlp_write_buffer": (_I, [_P, _I, _P, ctypes.c_size_t]),

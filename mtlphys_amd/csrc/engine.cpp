@@ -839,6 +839,28 @@ mtlp_status mtlp_exclusive_scan(mtlp_engine* e, mtlp_stream stream, const uint32
     return MTLP_OK;
 }
 
+mtlp_status mtlp_exclusive_scan_noalloc(mtlp_stream stream, const uint32_t* d_in, uint32_t* d_out, uint32_t n) {
+    if (n == 0) return MTLP_OK;
+    if (n > MTLP_NOALLOC_SCAN_MAX)
+        return fail(MTLP_E_RANGE, "scan length above MTLP_NOALLOC_SCAN_MAX (2^28) for the allocation-free scan");
+    if (!d_in || !d_out) return fail(MTLP_E_INVALID, "NULL buffer");
+    if ((reinterpret_cast<uintptr_t>(d_in) | reinterpret_cast<uintptr_t>(d_out)) & 15u)
+        return fail(MTLP_E_INVALID, "scan buffers must be 16-byte aligned");
+    static_assert(MTLP_NOALLOC_SCAN_MAX == kNoAllocScanMax, "header and kernel limits agree");
+    // the current device's copy of the scan's working memory (no allocation)
+    uint64_t* scratch = nullptr;
+    HIP_TRY(noalloc_scan_scratch(&scratch));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const uint32_t tiles = scan_tiles(n);
+    uint32_t* words = reinterpret_cast<uint32_t*>(scratch);  // [0] ticket, [1] error
+    uint64_t* state = scratch + 2;
+    // the last call's look-back words and ticket are cleared by a kernel (not
+    // a memset), so the call may also be captured into a graph
+    HIP_TRY(launch_fill_u32(words, 0u, 4 + 2 * (size_t)tiles, s));
+    HIP_TRY(launch_exclusive_scan(const_cast<uint32_t*>(d_in), d_out, n, false, state, words, 1u, words + 1, s));
+    return MTLP_OK;
+}
+
 mtlp_status mtlp_get_buffer(mtlp_engine* e, mtlp_buffer which, void** d_ptr, size_t* bytes) {
     MTLP_TRY(check_engine(e));
     if (!d_ptr) return fail(MTLP_E_INVALID, "d_ptr is NULL");

@@ -126,6 +126,11 @@ hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool z
                                  uint64_t* state, uint32_t* ticket, uint32_t epoch,
                                  uint32_t* err, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+// The allocation-free scan's working memory on the current device (a
+// file-scope device array in scan.hip): 4 u32 words, then
+// kNoAllocScanMax / kScanTile look-back words.
+constexpr uint32_t kNoAllocScanMax = 1u << 28;
+hipError_t noalloc_scan_scratch(uint64_t** out);
 
 // Neighbour list of one solver iterate: the density pass records each
 // slot's neighbours in its stencil-walk order, and the passes that follow

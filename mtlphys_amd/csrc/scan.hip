@@ -18,6 +18,8 @@
 // 2 inclusive prefix), [31:0] value.  The host bumps the epoch on every
 // launch, so words left by earlier launches read as "not ready" and the
 // state never needs clearing between launches.
+#include <mutex>
+
 #include "mtlp_device.h"
 
 namespace mtlp {
@@ -173,6 +175,37 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, size_t coun
 }  // namespace
 
 uint32_t scan_tiles(uint32_t n) { return (uint32_t)(((uint64_t)n + kScanTile - 1) / kScanTile); }
+
+// Working memory of the allocation-free scan (mtlp_exclusive_scan_noalloc):
+// 4 u32 words (tile ticket, error flag, 2 unused), then one look-back word
+// per tile of the longest scan it accepts, kNoAllocScanMax elements
+// (65,536 tiles: 512 KiB + 16 bytes).  Each GPU the library runs on holds
+// its own copy of this array until the process ends, and every call on a
+// GPU shares it.
+static __device__ __align__(256) uint64_t scratch[2 + kNoAllocScanMax / kScanTile];
+
+hipError_t noalloc_scan_scratch(uint64_t** out) {
+    // device address of `scratch` per device, looked up once per device with
+    // that device current (the array's name is only a host-side stand-in)
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static uint64_t* addr[kMaxDevices] = {};
+    *out = nullptr;
+    int dev = 0;
+    const hipError_t gd = hipGetDevice(&dev);
+    if (gd != hipSuccess) return gd;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[dev] = static_cast<uint64_t*>(p);
+    }
+    *out = addr[dev];
+    return hipSuccess;
+}
 
 hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool zero_input,
                                  uint64_t* state, uint32_t* ticket, uint32_t epoch,
