@@ -11,6 +11,10 @@ differentiable(n) the adjoint workload: the dam break at h = 0.04 (spacing
                   k 1e-4), 3 iterations, 40 steps with checkpoints every 8; loss
                   |centroid(x_T) - target|^2 + 0.1 mean((rho/rho0 - 1)^2) with
                   target = initial centroid + (0.2, 0, 0)
+unit_grid_params, uniform_points, clustered_points
+                  the isolated spatial hash: 4,194,304 points in the unit
+                  box's 128^3 grid, uniform or with 90% of them in 5% of the
+                  cells
 ensemble(...)     independent cube drops with N_s = round(exp(U[ln 2048,
                   ln 65536])), rest density U[5000, 7000] (times (0.1/h)^3),
                   gravity U[-12, -8], 3 iterations, drawn from numpy's PCG64
@@ -91,9 +95,67 @@ def make_ensemble(n_scenes=256, seed=3, device=0):
 
 
 def step_bytes(n, cells, iters):
-    """Compulsory bytes of one step: (152 + 44 I) N + (16 + 8 I) C."""
+    """Compulsory bytes of one step, B_step = (100 + 44 I) N + (16 + 8 I) C
+    (a vec3 counts 12 bytes, a u32 / f32 4): predict + hash + count 28 N +
+    4 C, scan 8 C, stable scatter + gather 20 N + 4 C, per iteration the
+    density pass 16 N + 4 C and the delta pass 28 N + 4 C, scatter back +
+    finalize 52 N."""
+    return (100 + 44 * iters) * n + (16 + 8 * iters) * cells
+
+
+def step_bytes_stored(n, cells, iters):
+    """Bytes of one step counting what this implementation stores as well:
+    float4 (16-byte) positions and velocities, the slot map, foam and lambda
+    outputs: (152 + 44 I) N + (16 + 8 I) C.  Reported as an extra, never as
+    the headline."""
     return (152 + 44 * iters) * n + (16 + 8 * iters) * cells
 
 
-__all__ = ["DT", "ADJ_STEPS", "ADJ_CKPT", "ADJ_SHIFT", "differentiable", "dam_break", "cube_drop", "ensemble_config", "make_ensemble", "step_bytes",
-           "Engine", "Batch"]
+# ---- the isolated spatial hash: 4M points in a 128^3 grid ------------------
+
+HASH_POINTS = 4 * 1024 * 1024
+HASH_CELLS = 128
+
+
+def unit_grid_params(cells=HASH_CELLS):
+    """The unit box [0, 1]^3 with cell = h = 1/cells (1/128 is exact)."""
+    p = C.default_params()
+    for a in range(3):
+        p.bounds_min[a] = 0.0
+        p.bounds_max[a] = 1.0
+    p.cell_size = 1.0 / cells
+    p.h = 1.0 / cells
+    return p
+
+
+def uniform_points(n=HASH_POINTS, seed=2):
+    """n points U[0, 1)^3 (FP32, w = 1)."""
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.random((n, 3), dtype=np.float32), np.ones((n, 1), np.float32)], 1)
+
+
+def clustered_points(n=HASH_POINTS, seed=2, cells=HASH_CELLS, frac_cells=0.05, frac_pts=0.9):
+    """n points of which frac_pts fall in frac_cells of the cells (chosen by
+    a seeded shuffle), the rest uniform; FP32, w = 1, in shuffled order."""
+    rng = np.random.default_rng(seed)
+    total = cells ** 3
+    hot = rng.permutation(total)[: int(round(frac_cells * total))]
+    n_hot = int(round(frac_pts * n))
+    pick = hot[rng.integers(0, len(hot), n_hot)]
+    cx, cy, cz = pick % cells, (pick // cells) % cells, pick // (cells * cells)
+    off = rng.random((n_hot, 3), dtype=np.float32)
+    hot_pts = (np.stack([cx, cy, cz], 1) + off * np.float32(0.999)) / np.float32(cells)
+    cold_pts = rng.random((n - n_hot, 3), dtype=np.float32)
+    pts = np.concatenate([hot_pts.astype(np.float32), cold_pts])[rng.permutation(n)]
+    return np.concatenate([pts, np.ones((n, 1), np.float32)], 1).astype(np.float32)
+
+
+def hash_bytes(n, cells):
+    """Compulsory bytes of hash + count + scan + stable scatter + gather:
+    28 N + 16 C."""
+    return 28 * n + 16 * cells
+
+
+__all__ = ["DT", "ADJ_STEPS", "ADJ_CKPT", "ADJ_SHIFT", "differentiable", "dam_break", "cube_drop", "ensemble_config",
+           "make_ensemble", "step_bytes", "step_bytes_stored", "HASH_POINTS", "HASH_CELLS", "unit_grid_params",
+           "uniform_points", "clustered_points", "hash_bytes", "Engine", "Batch"]

@@ -298,3 +298,117 @@ def test_adjoint_rejects_batches_and_bad_calls(mtlp):
             e.adjoint_rollout(DT * 2, 2, 1)
         with pytest.raises(mtlp.MtlpError):
             e.tape_read(mtlp.capi.TAPE_ITERATE, 0, 99)
+
+
+def _loss_seed(orc, op, xT, target):
+    """FP64 loss |centroid(x_T) - target|^2 + 0.1 mean(rho/rho0 - 1)^2 and
+    its gradient w.r.t. x_T (the density term through the oracle's density
+    adjoint)."""
+    xTd = np.asarray(xT, np.float64)
+    n = len(xTd)
+    c = xTd[:, :3].mean(axis=0) - target
+    rho = orc.density_at_d(xTd, op)
+    devn = rho / np.float64(op.rest_density) - 1.0
+    xb = np.zeros((n, 4))
+    xb[:, :3] = 2.0 * c / n
+    xb += orc.density_vjp_d(xTd, op, 0.1 * 2.0 * devn / (n * np.float64(op.rest_density)))
+    return float(c @ c + 0.1 * np.mean(devn ** 2)), xb
+
+
+def _rollout_loss_kinks(orc, op, x, v, steps, target):
+    """FP64 rollout from (x, v): loss and the branch signatures of every step."""
+    x, v = x.astype(np.float64), v.astype(np.float64)
+    sig = []
+    for _ in range(steps):
+        x, v, t = orc.step_d(x, v, op, DT)
+        sig.append(orc.step_kinks_d(t, op, DT))
+    return _loss_seed(orc, op, x, target)[0], np.stack(sig)
+
+
+def _gpu_grad(mtlp, e, p, x, v, target, steps, every):
+    import torch
+    from mtlphys_amd import autograd as AG
+    dev = torch.device("cuda")
+    e.upload_state(x, v)
+    x0 = torch.from_numpy(x).to(dev)
+    v0 = torch.from_numpy(v).to(dev).requires_grad_(True)
+    g = torch.tensor(list(p.gravity), dtype=torch.float32, requires_grad=True)
+    tgt = torch.tensor(target, dtype=torch.float32, device=dev)
+    xT, _ = AG.rollout(e, x0, v0, g, DT, steps, every)
+    loss = AG.centroid_density_loss(e, xT, tgt)
+    loss.backward()
+    return (float(loss.detach()), v0.grad.cpu().numpy().astype(np.float64), g.grad.numpy().astype(np.float64),
+            xT.detach().cpu().numpy())
+
+
+def test_differentiable_workload_full_length(mtlp, orc):
+    """The differentiable workload as specified: 65,536 particles, h = 0.04,
+    3 iterations, 40 steps with checkpoints every 8, loss |centroid(x_T) -
+    target|^2 + 0.1 mean(rho/rho0 - 1)^2.  dL/dv0 and dL/dg from the torch
+    harness (mtlp_rollout + mtlp_adjoint_rollout) against the FP64 oracle's
+    adjoint chained over the GPU's own 40 tapes: within 1e-3 relative (norm),
+    and along 32 seeded unit directions within 1e-3 of |dL/dv0|.
+
+    Central finite differences cannot check this horizon: on the FP64 oracle
+    |dL/dv0| is ~3e25 after 40 steps (~6e3 after 24), the rollout is chaotic
+    and its linear range lies far below any usable step.  FD is checked where
+    it is linear: the first checkpoint segment (8 steps), step 1e-4 along the
+    same 32 directions: the GPU gradient must match the FP64 oracle's FD
+    within 1e-3 of |dL/dv0|.  Kink crossings (particles whose branch
+    signature -- bin, stencil cell, clamp, cap, wall flag, gate -- differs
+    between the +-step runs in some step) are counted on the first four
+    directions and reported: every direction crosses a few, with FD agreeing
+    to ~1e-8 all the same, so no direction is left out."""
+    from mtlphys_amd import workloads as W
+    C = mtlp.capi
+    p, x, v, target = W.differentiable()
+    n, steps, every = len(x), W.ADJ_STEPS, W.ADJ_CKPT
+    op = orc.params(p)
+    print(f"differentiable workload: n={n} steps={steps} ckpt={every}", flush=True)
+    with _engine(mtlp, p, x, v) as e:
+        loss, gv, gg, xT = _gpu_grad(mtlp, e, p, x, v, target, steps, every)
+        assert e.error_flags() == 0
+        # the FP64 adjoint on the GPU's own trajectory, tape by tape from the end
+        e.upload_state(x, v)
+        e.tape_record(DT, steps)
+        np.testing.assert_array_equal(e.tape_read(C.TAPE_POSITIONS, steps).view(np.uint32), xT.view(np.uint32))
+        L_ref, xb = _loss_seed(orc, op, xT, target)
+        vb, rg = np.zeros((n, 4)), np.zeros(3)
+        sums = orc.adjoint_sums(xb, vb)
+        for t in reversed(range(steps)):
+            xb, vb, g1 = orc.step_vjp_d(_cpu_tape(mtlp, e, t, p.iterations), op, DT, xb, vb, sums)
+            rg += g1
+        assert e.error_flags() == 0
+        # the first checkpoint segment, for the finite differences
+        loss8, gv8, _, _ = _gpu_grad(mtlp, e, p, x, v, target, every, every)
+        assert e.error_flags() == 0
+    errs = dict(loss=abs(loss - L_ref) / L_ref, v0=_rel(gv, vb), gravity=_rel(gg, rg))
+    rng = np.random.default_rng(32)
+    dirs = []
+    for _ in range(32):
+        d = rng.standard_normal((n, 4))
+        d[:, 3] = 0
+        dirs.append(d / np.linalg.norm(d))
+    worst = max(abs(((gv - vb) * d).sum()) / np.linalg.norm(vb) for d in dirs)
+    print("40 steps vs FP64 adjoint on the GPU tapes:", errs, "worst direction", worst,
+          "|dL/dv0|", np.linalg.norm(vb), "dL/dg", gg, rg, flush=True)
+    assert errs["loss"] < 1e-4 and errs["v0"] < 1e-3 and errs["gravity"] < 1e-3, errs
+    assert worst < 1e-3, worst
+
+    eps = 1e-4
+    fd_errs = []
+    for d in dirs:
+        fd = (orc.rollout_loss(x, v + eps * d, op, DT, every, target)[0] -
+              orc.rollout_loss(x, v - eps * d, op, DT, every, target)[0]) / (2 * eps)
+        fd_errs.append(abs(fd - (gv8 * d).sum()) / np.linalg.norm(gv8))
+    # kink crossings of the first directions' +-eps runs: particles whose
+    # branch signature differs in some step (they do cross, and FD is
+    # unaffected at this size)
+    crossing = []
+    for d in dirs[:4]:
+        _, sp = _rollout_loss_kinks(orc, op, x, v + eps * d, every, target)
+        _, sm = _rollout_loss_kinks(orc, op, x, v - eps * d, every, target)
+        crossing.append(int(np.any(sp != sm, axis=(0, 2)).sum()))
+    print(f"8-step FD (step {eps}) vs GPU gradient over 32 directions: worst {max(fd_errs):.2e}, "
+          f"median {np.median(fd_errs):.2e}; kink-crossing particles in directions 0-3: {crossing}", flush=True)
+    assert max(fd_errs) < 1e-3, max(fd_errs)
