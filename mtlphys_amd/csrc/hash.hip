@@ -195,21 +195,17 @@ __device__ __forceinline__ void count_neighbors_slot(uint32_t s, const SceneDev&
             const uint32_t hi = min(cell_start[row + (uint32_t)x1 + 1], n);
             for (uint32_t j = lo; j < hi; ++j) {
                 const float4 q = sorted[j];
-                float d2;
-                {
-#pragma clang fp contract(off)
-                    const float dx = q.x - p.x, dy = q.y - p.y, dz = q.z - p.z;
-                    d2 = dx * dx + dy * dy + dz * dz;
-                }
-                cnt += (j != s && d2 < r2max) ? 1u : 0u;
+                const float d = nbr_key(q.x - p.x, q.y - p.y, q.z - p.z, r2max);
+                cnt += (j != s && d < 0.0f) ? 1u : 0u;
             }
         }
     nbr[sorted_index[s]] = cnt;
 }
 
 // countNeighbors (Spatial.metal:207-243), per sorted slot: neighbours j != s
-// in the 27-cell stencil of cell(p_s) with |p_j - p_s|^2 < h^2.  Same
-// arithmetic (no FMA) as the reference's dot(d, d).
+// in the 27-cell stencil of cell(p_s) with |p_j - p_s|^2 < h^2, by the
+// neighbour test of the stencil passes (nbr_key), so the counts equal the
+// iteration-0 density pass's.
 __global__ __launch_bounds__(kBlock) void k_count_neighbors(
     const float4* __restrict__ sorted, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, StepArgs a, uint32_t* __restrict__ nbr) {

@@ -31,6 +31,16 @@ __device__ __forceinline__ uint32_t key_of(const Cell3& c, const GridDev& g) {
     return (uint32_t)c.x + (uint32_t)c.y * g.dx + (uint32_t)c.z * g.dxy;
 }
 
+// The neighbour test of every stencil walk and of countNeighbors: |r|^2 <
+// h^2 (Constraints.metal:112-113, Spatial.metal:235-236: dot(r, r) < h^2),
+// evaluated as ((rx rx - h^2) + ry ry) + rz rz, each step one fused
+// multiply-add; the result's sign bit is set exactly for a neighbour.  Three
+// FMAs instead of three multiplies, two adds and a subtract; the CPU oracle
+// evaluates the same chain with fmaf.
+__device__ __forceinline__ float nbr_key(float rx, float ry, float rz, float h2) {
+    return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, -h2)));
+}
+
 // Scene of particle (or sorted slot) i: the last s with p_off[s] <= i.
 // With one scene the result is the constant 0, so scene fields are read
 // with uniform (scalar) loads.

@@ -51,8 +51,8 @@ __device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur
         for (uint32_t u = 0; u < 8; ++u) v[u] = q[k0 + u];
 #pragma unroll
         for (uint32_t u = 0; u < 8; ++u) {
-            // shift in the sign of r2 - h2 (exact in FP32): set when r2 < h2
-            const float d = dist2(pi.x - v[u].x, pi.y - v[u].y, pi.z - v[u].z) - h2;
+            // shift in the sign of the neighbour key: set for a neighbour
+            const float d = nbr_key(pi.x - v[u].x, pi.y - v[u].y, pi.z - v[u].z, h2);
             mask = __builtin_amdgcn_alignbit(mask, __float_as_uint(d), 31);
         }
     }
@@ -176,7 +176,7 @@ __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& p
 #pragma unroll 1
         for (uint32_t c = first > k0 ? a + 32u * (first - k0) : a; c < b; ++c) {
             const float4 q = cur[c];
-            if (c != s && dist2(pi.x - q.x, pi.y - q.y, pi.z - q.z) < sc.sd.h2) fn(c);
+            if (c != s && nbr_key(pi.x - q.x, pi.y - q.y, pi.z - q.z, sc.sd.h2) < 0.0f) fn(c);
         }
         k0 += nch;
     }
@@ -193,8 +193,10 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
     uint32_t K = 0, c = 0;
     for_each_chunk<true>(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
         const uint2 e = make_uint2(c0, mask);
+        // two stores under conditions of their own: as if / else if, the two
+        // destinations merge into one generic pointer (flat stores)
         if (K < kLdsChunks) le[K * lst] = e;
-        else if (K < kChunkScratch) nl.spill[(size_t)K * nl.stride + s] = e;
+        if (K - kLdsChunks < kChunkScratch - kLdsChunks) nl.spill[(size_t)K * nl.stride + s] = e;
         ++K;
         c += (uint32_t)__builtin_popcount(mask);
     });
@@ -615,7 +617,7 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
                 bool is_nbr = false;
                 if (j < rows.b[k] && j != s) {
                     const float4 q = cur[j];
-                    is_nbr = dist2(pi.x - q.x, pi.y - q.y, pi.z - q.z) - sd.h2 < 0.0f;
+                    is_nbr = nbr_key(pi.x - q.x, pi.y - q.y, pi.z - q.z, sd.h2) < 0.0f;
                 }
                 group(is_nbr, j);
             }

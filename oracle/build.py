@@ -16,8 +16,10 @@ SRC = [os.path.join(HERE, "pbf_oracle.c")]
 DEPS = SRC + [os.path.join(HERE, n) for n in ("pbf_oracle.h", "pbf_solver_impl.h", "pbf_adjoint_impl.h")]
 
 # -ffp-contract=off: predict and the cell hash are separately rounded FP32
-# operations (no FMA), which the GPU kernels reproduce bit for bit.
-CFLAGS = ["-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+# operations (no FMA), which the GPU kernels reproduce bit for bit.  -mfma:
+# the explicit fmaf of the neighbour test becomes the (correctly rounded)
+# hardware instruction instead of a slow library call; contraction stays off.
+CFLAGS = ["-O2", "-std=c11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-mfma",
           "-fno-fast-math", "-Wall", "-Wextra", "-Wno-unused-parameter"]
 
 

@@ -69,6 +69,17 @@ static void cell_coord_f32(const float p[3], const orc_grid* g, int c[3])
 }
 
 /* cellHashOf (Spatial.metal:41-43) */
+/* The neighbour test of the FP32 stencil walks and countNeighbors: the
+ * reference's dot(r, r) < h^2 (Spatial.metal:235-236, Constraints.metal:
+ * 112-113; Metal's fast math lets dot contract into fused multiply-adds) as
+ * ((r0 r0 - h^2) + r1 r1) + r2 r2, one correctly rounded fmaf per step; a
+ * negative result is a neighbour.  (Built with -mfma, fmaf is the hardware
+ * instruction.) */
+static float nbr_key_f32(float r0, float r1, float r2, float h2)
+{
+    return fmaf(r2, r2, fmaf(r1, r1, fmaf(r0, r0, -h2)));
+}
+
 static uint32_t key_of(const int c[3], const orc_grid* g)
 {
     return (uint32_t)c[0] + (uint32_t)c[1] * g->dim[0] + (uint32_t)c[2] * g->dim[0] * g->dim[1];
@@ -181,7 +192,7 @@ void orc_build_spatial_hash(const float* p4, uint32_t n, const orc_grid* g,
                         if (sorted_index[i] == gid) continue;
                         const float* q = sorted4 + 4 * (size_t)i;
                         const float d0 = q[0] - p[0], d1 = q[1] - p[1], d2 = q[2] - p[2];
-                        if (d0 * d0 + d1 * d1 + d2 * d2 < r2max) ++count;
+                        if (nbr_key_f32(d0, d1, d2, r2max) < 0.0f) ++count;
                     }
                 }
         neighbor_counts[gid] = count;
@@ -281,21 +292,27 @@ int orc_get_threads(void)
 #define CAT(a, b) CAT2(a, b)
 #define FN(name) CAT(CAT(name, _), SFX)
 
+/* FP64: the plain test on the exact square (the contract); FP32: the fused
+ * chain of the GPU kernels (nbr_key_f32). */
 #define REAL double
 #define SFX f64
 #define SQRT_OF(x) sqrt(x)
+#define NOT_NEIGHBOR(r, r2, h2) ((r2) >= (h2))
 #include "pbf_solver_impl.h"
 #undef REAL
 #undef SFX
 #undef SQRT_OF
+#undef NOT_NEIGHBOR
 
 #define REAL float
 #define SFX f32
 #define SQRT_OF(x) sqrtf(x)
+#define NOT_NEIGHBOR(r, r2, h2) (!(nbr_key_f32((r)[0], (r)[1], (r)[2], (h2)) < 0.0f))
 #include "pbf_solver_impl.h"
 #undef REAL
 #undef SFX
 #undef SQRT_OF
+#undef NOT_NEIGHBOR
 
 /* ---- FP64 rollout step and adjoint (uses the f64 helpers above) -------- */
 #include "pbf_adjoint_impl.h"

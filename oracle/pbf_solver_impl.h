@@ -4,7 +4,9 @@
  * with REAL = double / float and SFX = f64 / f32.
  *
  * Needs from the includer: REAL, SFX, SQRT_OF(x) = sqrt in REAL,
- * cell_coord_f32(), key_of(), and the CAT/FN macros.
+ * NOT_NEIGHBOR(r, r2, h2) = the neighbour test failing for the pair
+ * difference r (|r|^2 = r2), cell_coord_f32(), key_of(), and the CAT/FN
+ * macros.
  */
 
 /* Derived constants of writePBFParams (Engine.cpp:551-572), in REAL. */
@@ -123,7 +125,7 @@ void CAT(orc_density_lambda_, SFX)(const REAL* sx4, const uint32_t* cell_start,
                         const REAL* pj = sx4 + 4 * (size_t)j;
                         const REAL r[3] = { pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2] };
                         const REAL r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-                        if (r2 >= k.h2) continue;
+                        if (NOT_NEIGHBOR(r, r2, k.h2)) continue;
                         ++cnt;
                         density += FN(poly6)(r2, k.h2, k.poly6_norm);
                         REAL gr[3];
@@ -169,7 +171,7 @@ void CAT(orc_apply_delta_, SFX)(const REAL* sx4, const REAL* lambda_sorted,
                         const REAL* pj = sx4 + 4 * (size_t)j;
                         const REAL r[3] = { pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2] };
                         const REAL r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-                        if (r2 >= k.h2) continue;
+                        if (NOT_NEIGHBOR(r, r2, k.h2)) continue;
                         const REAL lam_j = lambda_sorted[j];
                         REAL gr[3];
                         FN(spiky_grad)(r, SQRT_OF(r2), k.h, k.spiky_norm, gr);

@@ -128,11 +128,81 @@ def test_neighbor_counts_reference_cases(orc):
     assert nb[13] == 6 and nb[0] < nb[13] and nb[26] < nb[13]
 
 
+def fmaf32(a, b, c):
+    """IEEE fused multiply-add in float32 (one rounding of a*b + c), from
+    float64: a*b is exact in float64, the sum is formed exactly as s + e
+    (TwoSum), and s is rounded to float32; s + e rounds differently from s
+    only when s is itself a float32 midpoint and e != 0, which picks the
+    neighbour on e's side."""
+    a, b, c = (np.asarray(t, np.float32).astype(np.float64) for t in (a, b, c))
+    p = a * b
+    s = p + c
+    bv = s - p
+    e = (p - (s - bv)) + (c - bv)
+    f = s.astype(np.float32)
+    f64 = f.astype(np.float64)
+    other = np.nextafter(f, np.where(s > f64, np.float32(np.inf), np.float32(-np.inf))).astype(np.float32)
+    mid = (f64 + other.astype(np.float64)) * 0.5
+    tie = (s == mid) & (e != 0) & (s != f64)
+    up = np.maximum(f, other)
+    down = np.minimum(f, other)
+    return np.where(tie, np.where(e > 0, up, down), f).astype(np.float32)
+
+
+def nbr_key32(d0, d1, d2, h2):
+    """The neighbour test's key ((d0 d0 - h^2) + d1 d1) + d2 d2, fused."""
+    return fmaf32(d2, d2, fmaf32(d1, d1, fmaf32(d0, d0, -np.float32(h2))))
+
+
 def _brute_counts(pts, h2):
     d = pts[:, None, :3].astype(np.float32) - pts[None, :, :3].astype(np.float32)
-    r2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
-    np.fill_diagonal(r2, np.inf)
-    return (r2 < h2).sum(1)
+    key = nbr_key32(d[..., 0], d[..., 1], d[..., 2], h2)
+    np.fill_diagonal(key, np.inf)
+    return (key < 0).sum(1)
+
+
+def test_fmaf32_emulation():
+    """fmaf32 against exact rational arithmetic on random and tie cases."""
+    from fractions import Fraction
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal(400).astype(np.float32)
+    b = rng.standard_normal(400).astype(np.float32)
+    c = (rng.standard_normal(400) * 1e-3).astype(np.float32)
+    # constructed ties: (1 + 2^-12)^2 = 1 + 2^-11 + 2^-24 is a float32
+    # midpoint; adding +-2^-80 (below float64's resolution there) must break
+    # the tie towards its sign, adding 0 leaves it to round-half-even
+    t = np.float32(1.0 + 2.0 ** -12)
+    a = np.concatenate([a, [t, t, t, -t]]).astype(np.float32)
+    b = np.concatenate([b, [t, t, t, t]]).astype(np.float32)
+    c = np.concatenate([c, [2.0 ** -80, -(2.0 ** -80), 0.0, 2.0 ** -80]]).astype(np.float32)
+    got = fmaf32(a, b, c)
+    for x, y, z, g in zip(a, b, c, got):
+        exact = Fraction(float(x)) * Fraction(float(y)) + Fraction(float(z))
+        lo = np.float32(float(exact))
+        cands = [lo, np.nextafter(lo, np.float32(np.inf)), np.nextafter(lo, np.float32(-np.inf))]
+        best = min(cands, key=lambda t: (abs(Fraction(float(t)) - exact),
+                                          int(np.float32(t).view(np.uint32)) & 1))
+        assert g == best, (x, y, z, g, best)
+
+
+def test_neighbor_key_matches_exact_arithmetic():
+    """The fused key's sign is the exact sign of |r|^2 - h^2 (extended
+    precision) for every pair except those within a few float32 ulps of the
+    sphere (random pairs, and pairs placed on the sphere)."""
+    rng = np.random.default_rng(13)
+    h = np.float32(0.02)
+    h2 = h * h
+    d = (rng.uniform(-1.2, 1.2, (200000, 3)) * h).astype(np.float32)
+    u = rng.standard_normal((20000, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    ring = (u * float(h) * (1.0 + rng.uniform(-1e-6, 1e-6, (20000, 1)))).astype(np.float32)
+    d = np.concatenate([d, ring])
+    key = nbr_key32(d[:, 0], d[:, 1], d[:, 2], h2)
+    dl = d.astype(np.longdouble)
+    exact = (dl[:, 0] * dl[:, 0] + dl[:, 1] * dl[:, 1] + dl[:, 2] * dl[:, 2]) - np.longdouble(h2)
+    near = np.abs(exact) <= 4 * np.spacing(h2)
+    assert (np.sign(key[~near]) == np.sign(exact[~near]).astype(np.float32)).all()
+    assert near.sum() > 0 and (~near).sum() > 200000
 
 
 def test_neighbor_sets_brute_force(orc):
