@@ -641,19 +641,19 @@ hipError_t launch_apply_delta_tape(const float4* cur, const uint32_t* cell_start
     if (a.n == 0) return hipSuccess;
     switch (block) {  // the heavy queue's workgroups in front
         case 64:
-            hipLaunchKernelGGL(k_apply_delta_tape<64>, dim3(blocks_for(a.n, 64) + heavy_blocks<64>() + nl.lanes_cap / 64), dim3(64), 0,
+            hipLaunchKernelGGL(k_apply_delta_tape<64>, dim3(blocks_for(a.n, 64) + heavy_blocks<64>() + split_blocks<64>()), dim3(64), 0,
                                s, cur, cell_start, a, next, rr, nl);
             break;
         case 128:
-            hipLaunchKernelGGL(k_apply_delta_tape<128>, dim3(blocks_for(a.n, 128) + heavy_blocks<128>() + nl.lanes_cap / 128), dim3(128),
+            hipLaunchKernelGGL(k_apply_delta_tape<128>, dim3(blocks_for(a.n, 128) + heavy_blocks<128>() + split_blocks<128>()), dim3(128),
                                0, s, cur, cell_start, a, next, rr, nl);
             break;
         case 512:
-            hipLaunchKernelGGL(k_apply_delta_tape<512>, dim3(blocks_for(a.n, 512) + heavy_blocks<512>() + nl.lanes_cap / 512), dim3(512),
+            hipLaunchKernelGGL(k_apply_delta_tape<512>, dim3(blocks_for(a.n, 512) + heavy_blocks<512>() + split_blocks<512>()), dim3(512),
                                0, s, cur, cell_start, a, next, rr, nl);
             break;
         default:
-            hipLaunchKernelGGL(k_apply_delta_tape<256>, dim3(blocks_for(a.n, 256) + heavy_blocks<256>() + nl.lanes_cap / 256), dim3(256),
+            hipLaunchKernelGGL(k_apply_delta_tape<256>, dim3(blocks_for(a.n, 256) + heavy_blocks<256>() + split_blocks<256>()), dim3(256),
                                0, s, cur, cell_start, a, next, rr, nl);
             break;
     }

@@ -433,6 +433,35 @@ __device__ __forceinline__ void add_term(const DeltaTerm& t, double& dx, double&
     dz = fma(t.c, t.rz, dz);
 }
 
+// The delta pass's sum R = sum_j c_j r_j in walk order, in blocks of
+// kSumBlock neighbours: each block is summed from zero with fused
+// multiply-adds, and the sums of the non-empty blocks are added in block
+// order.  Every path that forms R (a lane per slot, a wave per slot, the
+// lanes of a heavy slot taking a block each) keeps this order, so R has one
+// set of bits whichever path runs; splitting a slot into blocks cuts its
+// dependency chain from its neighbour count to a block.
+constexpr uint32_t kSumBlock = 32;
+struct DeltaSum {
+    double dx = 0.0, dy = 0.0, dz = 0.0;  // sum of the finished blocks
+    double bx = 0.0, by = 0.0, bz = 0.0;  // the current block
+    uint32_t n = 0;                       // terms in the current block
+    __device__ __forceinline__ void add(const DeltaTerm& t) {
+        add_term(t, bx, by, bz);
+        if (++n == kSumBlock) fold();
+    }
+    __device__ __forceinline__ void fold() {
+#pragma clang fp contract(off)
+        dx += bx;
+        dy += by;
+        dz += bz;
+        bx = by = bz = 0.0;
+        n = 0;
+    }
+    __device__ __forceinline__ void finish() {
+        if (n) fold();
+    }
+};
+
 // densityLambdaSorted for slot s: lambda_s into cur[s].w.  With kTape the
 // constraint-gradient sums the adjoint needs go to aux[s] = (sum_j grad W,
 // D) with D = (|sum_j grad W|^2 + sum_j |grad W|^2) / rho0^2, so that
@@ -555,11 +584,12 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
     const SolverDev& sd = sc.sd;
     const float4 pi = cur[s];
     const double lam_d = pi.w;
-    double dx = 0.0, dy = 0.0, dz = 0.0;
+    DeltaSum r;
     for_each_neighbor_of<kList, true>(
         s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
-        [&](uint32_t, const float4& q) { add_term(delta_coef(pi, lam_d, q, sd), dx, dy, dz); });
-    finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
+        [&](uint32_t, const float4& q) { r.add(delta_coef(pi, lam_d, q, sd)); });
+    r.finish();
+    finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere);
 }
 
 __device__ __forceinline__ double readlane_d(double v, int lane) {
@@ -585,15 +615,14 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
     const uint32_t lane = __lane_id();
     const float4 pi = cur[s];
     const double lam_d = pi.w;
-    double dx = 0.0, dy = 0.0, dz = 0.0;  // the same in every lane
+    DeltaSum r;  // the same in every lane
     // one group of candidates, lane l's being j (a neighbour when is_nbr)
     auto group = [&](bool is_nbr, uint32_t j) {
         DeltaTerm t{0.0, 0.0, 0.0, 0.0};
         if (is_nbr) t = delta_coef(pi, lam_d, cur[j], sd);
         for (uint64_t bal = __ballot(is_nbr); bal; bal &= bal - 1ull) {
             const int l = __builtin_ctzll(bal);
-            add_term(DeltaTerm{readlane_d(t.c, l), readlane_d(t.rx, l), readlane_d(t.ry, l), readlane_d(t.rz, l)},
-                     dx, dy, dz);
+            r.add(DeltaTerm{readlane_d(t.c, l), readlane_d(t.rx, l), readlane_d(t.ry, l), readlane_d(t.rz, l)});
         }
     };
     const uint32_t K = nl.count[s] & 0xffffu;
@@ -623,23 +652,147 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
             }
         }
     }
-    if (lane == 0) finish_delta<kTape>(s, pi, sd, dx, dy, dz, next, rr, sphere);
+    r.finish();
+    if (lane == 0) finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere);
+}
+
+// The walk of block B (neighbours kSumBlock B .. kSumBlock (B + 1) - 1 in
+// walk order) of a slot whose K <= kChunkScratch chunks are in nl.spill:
+// its sum from zero into (bx, by, bz); returns false when the walk has no
+// such neighbours.  The chunks' set bits are counted four chunk loads at a
+// time to find the block's first neighbour.
+__device__ __forceinline__ bool delta_block_sum(uint32_t s, uint32_t K, uint32_t B, const float4& pi, double lam_d,
+                                                const SolverDev& sd, const float4* __restrict__ cur,
+                                                const NbrList& nl, double& bx, double& by, double& bz) {
+    const uint32_t target = kSumBlock * B;
+    const uint2* sp = nl.spill + s;
+    const size_t st = nl.stride;
+    uint32_t k = 0, acc = 0, m = 0, base = 0;
+    bool found = false;
+#pragma unroll 1
+    for (uint32_t k0 = 0; k0 < K && !found; k0 += 4) {
+        uint2 e[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) e[u] = k0 + u < K ? sp[(size_t)(k0 + u) * st] : make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t pc = (uint32_t)__builtin_popcount(e[u].y);
+            if (!found && k0 + u < K) {
+                if (acc + pc > target) {  // the chunk holding the block's first neighbour
+                    found = true;
+                    m = e[u].y;
+                    base = e[u].x;
+                    k = k0 + u;
+                } else {
+                    acc += pc;
+                }
+            }
+        }
+    }
+    if (!found) return false;
+    for (uint32_t d = target - acc; d; --d) m &= m - 1u;  // drop the neighbours of earlier blocks
+    bx = by = bz = 0.0;
+    uint32_t left = kSumBlock;
+#pragma unroll 1
+    while (left) {
+        while (m == 0u && k + 1u < K) {
+            ++k;
+            const uint2 e = sp[(size_t)k * st];
+            base = e.x;
+            m = e.y;
+        }
+        if (m == 0u) break;
+        // up to four neighbours per trip, their loads in flight together
+        uint32_t j[4];
+        uint32_t got = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            j[u] = s;
+            if (m != 0u && got < left) {
+                j[u] = base + (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                ++got;
+            }
+        }
+        const float4 q0 = cur[j[0]], q1 = cur[j[1]], q2 = cur[j[2]], q3 = cur[j[3]];
+        add_term(delta_coef(pi, lam_d, q0, sd), bx, by, bz);
+        if (got > 1) add_term(delta_coef(pi, lam_d, q1, sd), bx, by, bz);
+        if (got > 2) add_term(delta_coef(pi, lam_d, q2, sd), bx, by, bz);
+        if (got > 3) add_term(delta_coef(pi, lam_d, q3, sd), bx, by, bz);
+        left -= got;
+    }
+    return true;
+}
+
+// applyDeltaSorted for the slots of the heavy-lane queue (more than kMaxList
+// neighbours in at most kChunkScratch chunks): kSplitLanes lanes per slot,
+// lane b of a slot summing blocks b, b + kSplitLanes, ... of its walk, one
+// block per round; after each round the slot's first lane adds the round's
+// block sums in block order, so the sum is DeltaSum's bit for bit.  q_ok is
+// false for lanes past the queue: they take part in the wave's exchanges
+// and compute nothing.  Every lane of the wave must be active.
+constexpr uint32_t kSplitLanes = 8;
+
+template <bool kTape>
+__device__ __forceinline__ void apply_delta_split(uint32_t s, bool q_ok, const StepArgs& a,
+                                                  const float4* __restrict__ cur, float4* __restrict__ next,
+                                                  float4* __restrict__ rr, const NbrList& nl) {
+    const uint32_t lane = __lane_id();
+    const uint32_t b = lane % kSplitLanes;
+    const uint32_t K = q_ok ? nl.count[s] & 0xffffu : 0u;
+    const float4 pi = q_ok ? cur[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double lam_d = pi.w;
+    double dx = 0.0, dy = 0.0, dz = 0.0;  // the sum so far, in the slot's first lane
+#pragma unroll 1
+    for (uint32_t round = 0;; ++round) {
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        bool has = false;
+        if (q_ok) {
+            with_scene(s, a, [&](const SceneDev& sc) {
+                has = delta_block_sum(s, K, b + kSplitLanes * round, pi, lam_d, sc.sd, cur, nl, bx, by, bz);
+            });
+        }
+        if (__ballot(has) == 0ull) break;
+        // the slot's first lane adds its lanes' block sums in block order
+        if (has && b == 0u) {
+            dx += bx;
+            dy += by;
+            dz += bz;
+        }
+#pragma unroll
+        for (uint32_t o = 1; o < kSplitLanes; ++o) {
+            const double ox = __shfl(bx, (int)(lane + o)), oy = __shfl(by, (int)(lane + o)),
+                         oz = __shfl(bz, (int)(lane + o));
+            const bool oh = __shfl((int)has, (int)(lane + o)) != 0;
+            if (b == 0u && oh) {
+                dx += ox;
+                dy += oy;
+                dz += oz;
+            }
+        }
+    }
+    if (q_ok && b == 0u) {
+        with_scene(s, a, [&](const SceneDev& sc) { finish_delta<kTape>(s, pi, sc.sd, dx, dy, dz, next, rr, a.sphere); });
+    }
 }
 
 // A delta launch over n slots with lists: workgroups [0, heavy_blocks<kB>)
 // come first in dispatch order and run the heavy queue, a wave per queued
-// slot (grid-stride over the queue, kHeavyWaves waves); the next lanes_cap
-// / kB walk the heavy-lane queue, a slot per lane (those past its count
-// exit at once), unless it holds at most kLanesByWaves slots (then the heavy
-// waves take them after their own); then, with an order to write, one
+// slot (grid-stride over the queue, kHeavyWaves waves); the next
+// split_blocks<kB> run the heavy-lane queue, kSplitLanes lanes per slot
+// (grid-stride, kSplitWaves waves); then, with an order to write, one
 // workgroup per XCD range for it; the others take the slots.
 constexpr uint32_t kHeavyWaves = 2048;
-// heavy-lane queues up to this long are run by the heavy waves (8 slots each)
-constexpr uint32_t kLanesByWaves = 8 * kHeavyWaves;
+constexpr uint32_t kSplitWaves = 4096;
 
 template <uint32_t kB>
 constexpr uint32_t heavy_blocks() {
     return kHeavyWaves / (kB / 64u);
+}
+
+template <uint32_t kB>
+constexpr uint32_t split_blocks() {
+    return kSplitWaves / (kB / 64u);
 }
 
 // Times a stencil workgroup's tile (its first wave, from the start to the
@@ -708,45 +861,41 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
                                                   float4* __restrict__ next, float4* __restrict__ rr,
                                                   const NbrList& nl) {
     constexpr uint32_t hb = heavy_blocks<kB>();
-    // a short heavy-lane queue goes to the heavy waves too: one lane alone
-    // walks a heavy slot at a memory latency per four neighbours, which
-    // outlasts the pass unless many such lanes share it
-    const uint32_t l_end = min(*nl.lanes_count, nl.lanes_cap);
-    const bool lanes_by_waves = l_end <= kLanesByWaves;
+    constexpr uint32_t lb = split_blocks<kB>();
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kB / 64u) + threadIdx.x / 64u));
     if (blockIdx.x < hb) {
-        const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * (kB / 64u) + threadIdx.x / 64u));
         const uint32_t h_end = min(*nl.heavy_count, kHeavyCap);
-        const uint32_t end = h_end + (lanes_by_waves ? l_end : 0u);
-        for (uint32_t h = wave; h < end; h += kHeavyWaves) {
-            const uint32_t q = h < h_end ? nl.heavy[h] : nl.lanes[h - h_end];  // h is wave-uniform
-            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
+        for (uint32_t h = wave; h < h_end; h += kHeavyWaves) {
+            const uint32_t s = (uint32_t)__builtin_amdgcn_readfirstlane((int)nl.heavy[h]);  // h is wave-uniform
             with_scene(s, a, [&](const SceneDev& sc) {
                 apply_delta_heavy<kTape>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
             });
         }
         return;
     }
-    const uint32_t lb = nl.lanes_cap / kB;
-    uint32_t s;
     if (blockIdx.x < hb + lb) {
-        const uint32_t h = (blockIdx.x - hb) * kB + threadIdx.x;
-        if (lanes_by_waves || h >= l_end) return;
-        s = nl.lanes[h];
-    } else {
-        const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
-        const uint32_t nbt = gridDim.x - hb - lb - ob;
-        if (blockIdx.x < hb + lb + ob) {
-            order_range(nl, blockIdx.x - hb - lb, nbt);
-            return;
+        // 64 / kSplitLanes queued slots per wave and trip
+        const uint32_t l_end = min(*nl.lanes_count, nl.lanes_cap);
+        constexpr uint32_t kPerWave = 64u / kSplitLanes;
+        for (uint32_t q0 = (wave - hb * (kB / 64u)) * kPerWave; q0 < l_end; q0 += kSplitWaves * kPerWave) {
+            const uint32_t q = q0 + __lane_id() / kSplitLanes;
+            const bool q_ok = q < l_end;
+            apply_delta_split<kTape>(q_ok ? nl.lanes[q] : 0u, q_ok, a, cur, next, rr, nl);
         }
-        const uint32_t pos = xcd_block(blockIdx.x - hb - lb - ob, nbt);
-        // the tiles in the density pass's order (heaviest first), else in place
-        const uint32_t tile =
-            nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
-        s = tile * kB + threadIdx.x;
-        if (!delta_takes_slot(s, a.n, nl)) return;
+        return;
     }
-    // one copy of the per-slot walk for heavy-lane and ordinary workgroups
+    const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
+    const uint32_t nbt = gridDim.x - hb - lb - ob;
+    if (blockIdx.x < hb + lb + ob) {
+        order_range(nl, blockIdx.x - hb - lb, nbt);
+        return;
+    }
+    const uint32_t pos = xcd_block(blockIdx.x - hb - lb - ob, nbt);
+    // the tiles in the density pass's order (heaviest first), else in place
+    const uint32_t tile =
+        nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
+    const uint32_t s = tile * kB + threadIdx.x;
+    if (!delta_takes_slot(s, a.n, nl)) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
     });

@@ -167,7 +167,7 @@ template <uint32_t kB>
 void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
                    hipStream_t s) {
     // with lists, the heavy queue's workgroups in front
-    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() + nl.lanes_cap / kB : 0u) +
+    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() + split_blocks<kB>() : 0u) +
                     (nl.tile_order_next ? kNumXcd : 0u)),
         block(kB);
     if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
