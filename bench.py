@@ -177,10 +177,19 @@ def batched_leg(steps, warmup, stream, torch):
     b = W.make_ensemble()
     n = b.particle_count
     block = b.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"))
+    pull = b.launch_config()[1]
     b.set_graph(True)
     for _ in range(warmup):
         b.step(DT, stream)
     t_batch = _time_steps(b, steps, stream, torch)
+    # one more step, eager, with per-kernel events: us per stage
+    b.set_graph(False)
+    b.set_profiling(True)
+    b.step(DT, stream)
+    stage_us = {}
+    for name, ms in b.stage_times():
+        stage_us[name] = round(stage_us.get(name, 0.0) + ms * 1e3, 1)
+    b.set_profiling(False)
     _, co = b.offsets()
     iters = b.scene_params[0].iterations
     bytes_batch = sum(W.step_bytes(c, int(co[s + 1] - co[s]), iters) for s, c in enumerate(b.counts))
@@ -198,10 +207,11 @@ def batched_leg(steps, warmup, stream, torch):
     bytes_single = W.step_bytes(n, single.total_cells, p.iterations)
     single.close()
     gbs_b, gbs_s = bytes_batch / t_batch / 1e9, bytes_single / t_single / 1e9
-    return {"scenes": 256, "particles": n, "iterations": iters, "stencil_block": block,
+    return {"scenes": 256, "particles": n, "iterations": iters, "stencil_block": block, "tile_pull": pull,
             "ms_per_step": round(t_batch * 1e3, 4), "value": round(gbs_b, 2), "unit": "GB/s",
             "single_scene_ms_per_step": round(t_single * 1e3, 4), "single_scene_gbs": round(gbs_s, 2),
             "ratio_to_single_scene": round(gbs_b / gbs_s, 3), "error_flags": flags,
+            "next_step_stage_us": stage_us,
             "note": "one launch per stage for all scenes, HIP-graph replay; the single scene is one "
                     "cube drop with the same total particle count; GB/s of the compulsory bytes"}
 
@@ -496,6 +506,7 @@ def main():
     # warmed-up state (the engine's autotuner times a few steps per size and
     # restores the state)
     block = eng.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"), stream)
+    pull = eng.launch_config()[1]
     warm_state = (params, eng.positions_buffer().read(), eng.velocities_buffer().read())
 
     # timed region: K steps, HIP events after every kernel on the launch stream
@@ -585,7 +596,7 @@ def main():
         "data": "synthetic: PCG32-jittered lattice dam break, seed 1",
         "config": {"workload": "dam_break_1m", "particles": N_PARTICLES, "h": params.h, "cells": cells,
                    "grid": list(mtlp.grid_dims(params)), "iterations": iters, "dt": DT, "stencil_block": block,
-                   "alg_bytes_per_step": b_step,
+                   "tile_pull": pull, "alg_bytes_per_step": b_step,
                    "stored_bytes_per_step": step_bytes_stored(N_PARTICLES, cells, iters)},
         "roofline": roofline,
         "roofline_valu": roofline_valu,

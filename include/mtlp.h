@@ -248,16 +248,29 @@ mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n
  * graph and replays it (the look-back words of the scan are cleared by a
  * kernel inside the graph).  Profiling is ignored while graphs are on. */
 mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
-/* Times eight steps at each stencil workgroup size (64, 128, 256, 512),
- * two passes over the sizes, from the current state (restoring it
- * afterwards) and keeps the size with the fastest pass;
- * with cache_path, reads a previous choice for the same batch shape from
- * that text file or appends the new one.  *block receives the chosen size
- * (may be NULL). */
+/* Times eight steps at each launch configuration of the stencil kernels
+ * (workgroup size 64, 128, 256 or 512 times tiles per pull 0, 1 or 4, see
+ * mtlp_set_tile_pull), two passes over them, from the current state, and
+ * keeps the configuration with the fastest pass.  Afterwards every buffer a
+ * caller can read (positions, velocities, previous positions, cell hashes,
+ * cell offsets, sorted index and positions, neighbour counts, lambdas,
+ * foam) holds what it held before the call.  With cache_path, reads a
+ * previous choice for the same batch shape from that text file or appends
+ * the new one.  *block receives the chosen workgroup size (may be NULL). */
 mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const char* cache_path,
                           uint32_t* block);
 /* Workgroup size of the stencil kernels (64, 128, 256 or 512). */
 mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
+/* Launch mode of the stencil passes (no reference counterpart): 0 (the
+ * default) launches one workgroup per tile of slots, taken in dispatch
+ * order; k in 1..64 launches persistent workgroups (16 waves per compute
+ * unit) that take k tiles at a time from one counter per XCD, each XCD
+ * working through its own range of the tile order.  Results are the same
+ * bits either way. */
+mtlp_status mtlp_set_tile_pull(mtlp_engine* e, uint32_t tiles_per_pull);
+/* The stencil kernels' current workgroup size and tiles per pull (either
+ * pointer may be NULL). */
+mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull);
 /* Neighbour lists (on by default): each iteration's density pass records
  * up to 96 neighbours per particle, or the stencil chunks that hold them
  * (644 B of device memory per particle), and the delta pass walks them

@@ -221,9 +221,19 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(hipMemset(e->heavy_count, 0, 4 * sizeof(uint32_t)));
     HIP_TRY(dev_alloc(&e->lanes, 2 * (size_t)lanes_cap_for(na)));
     HIP_TRY(dev_alloc(&e->tile_plan, (na + 63) / 64));
+    HIP_TRY(dev_alloc(&e->tile_ctr, 2 * (size_t)kTileCtrs));
+    HIP_TRY(hipMemset(e->tile_ctr, 0, 2 * kTileCtrs * sizeof(uint32_t)));
     HIP_TRY(dev_alloc(&e->tile_order[0], (na + 63) / 64));
     HIP_TRY(dev_alloc(&e->tile_order[1], (na + 63) / 64));
     e->tile_order_tiles[0] = e->tile_order_tiles[1] = 0;
+    {
+        // both orders start as the identity, so a buffer marked valid always
+        // holds a permutation of the tiles
+        std::vector<uint32_t> iota((na + 63) / 64);
+        for (size_t t = 0; t < iota.size(); ++t) iota[t] = (uint32_t)t;
+        HIP_TRY(hipMemcpy(e->tile_order[0], iota.data(), sizeof(uint32_t) * iota.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->tile_order[1], iota.data(), sizeof(uint32_t) * iota.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMemset(e->x, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->v, 0, sizeof(float4) * na));
     HIP_TRY(hipMemset(e->prev, 0, sizeof(float4) * na));
@@ -364,9 +374,19 @@ mtlp_status capture_graph(mtlp_engine* e, float dt) {
     e->drop_graph();
     if (!e->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&e->capture_stream, hipStreamNonBlocking));
     HIP_TRY(hipStreamBeginCapture(e->capture_stream, hipStreamCaptureModeThreadLocal));
+    // Capturing records the kernels without running them, so the tile
+    // orders it marks as written are not: the host's tile-order state is
+    // put back afterwards, whether the capture succeeds or fails (the graph
+    // keeps the buffers it was captured with; every buffer it writes holds
+    // a permutation of the tiles).
+    const uint32_t order_cur = e->tile_order_cur;
+    const uint32_t order_tiles[2] = {e->tile_order_tiles[0], e->tile_order_tiles[1]};
     e->capturing = true;
     const mtlp_status st = enqueue_step(e, e->capture_stream, dt);
     e->capturing = false;
+    e->tile_order_cur = order_cur;
+    e->tile_order_tiles[0] = order_tiles[0];
+    e->tile_order_tiles[1] = order_tiles[1];
     hipGraph_t graph = nullptr;
     const hipError_t end = hipStreamEndCapture(e->capture_stream, &graph);
     if (st != MTLP_OK) {
@@ -389,6 +409,10 @@ mtlp_status new_engine(int hip_device, mtlp_engine** out) {
     if (!e) return fail(MTLP_E_NOMEM, "host allocation failed");
     e->device = hip_device;
     DeviceGuard g(hip_device);
+    int cus = 0;
+    if (g.ok && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device) == hipSuccess &&
+        cus > 0)
+        e->cus = (uint32_t)cus;
     if (!g.ok || hipMalloc(reinterpret_cast<void**>(&e->misc), 4 * sizeof(uint32_t)) != hipSuccess ||
         hipMemset(e->misc, 0, 4 * sizeof(uint32_t)) != hipSuccess) {
         delete e;
@@ -699,19 +723,45 @@ mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block) {
     return MTLP_OK;
 }
 
+mtlp_status mtlp_set_tile_pull(mtlp_engine* e, uint32_t tiles_per_pull) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (tiles_per_pull > kMaxTilePull) return fail(MTLP_E_RANGE, "tiles per pull above 64");
+    if (e->tile_pull == tiles_per_pull) return MTLP_OK;
+    // the counters are left as the last persistent launch of each pass left
+    // them: zero them for the next
+    if (e->tile_ctr) {
+        DeviceGuard g(e->device);
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemset(e->tile_ctr, 0, 2 * kTileCtrs * sizeof(uint32_t)));
+    }
+    e->drop_graph();
+    e->tile_pull = tiles_per_pull;
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (block) *block = e->stencil_block;
+    if (tiles_per_pull) *tiles_per_pull = e->tile_pull;
+    return MTLP_OK;
+}
+
 mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const char* cache_path,
                           uint32_t* block) {
     MTLP_TRY(check_engine(e));
     if (!(dt > 0.0f) || !std::isfinite(dt)) return fail(MTLP_E_INVALID, "dt must be > 0");
     std::ostringstream key;
-    key << "n=" << e->n << ",scenes=" << e->scene_params.size() << ",cells=" << e->cells
+    key << "launch,n=" << e->n << ",scenes=" << e->scene_params.size() << ",cells=" << e->cells
         << ",iters=" << e->iterations();
     if (cache_path) {
         std::ifstream in(cache_path);
-        std::string k;
-        uint32_t b;
-        while (in >> k >> b) {
-            if (k == key.str() && mtlp_set_stencil_block(e, b) == MTLP_OK) {
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            uint32_t b = 0, pull = 0;
+            if (ls >> k >> b >> pull && k == key.str() && mtlp_set_stencil_block(e, b) == MTLP_OK &&
+                mtlp_set_tile_pull(e, pull) == MTLP_OK) {
                 if (block) *block = b;
                 return MTLP_OK;
             }
@@ -723,78 +773,106 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
     }
     DeviceGuard g(e->device);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // time on a copy of the state, then put it back
-    float4 *sx = nullptr, *sv = nullptr, *sp = nullptr;
-    HIP_TRY(dev_alloc(&sx, e->n));
-    HIP_TRY(dev_alloc(&sv, e->n));
-    HIP_TRY(dev_alloc(&sp, e->n));
-    HIP_TRY(hipMemcpyAsync(sx, e->x, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemcpyAsync(sv, e->v, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
-    HIP_TRY(hipMemcpyAsync(sp, e->prev, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s));
+    // The tuning steps run from a copy of the state and every buffer a
+    // caller can read is put back afterwards: positions, velocities and
+    // previous positions, and the last step's outputs (cell hashes and
+    // offsets, sorted index and positions, neighbour counts, lambdas, foam).
+    const size_t n = e->n, c1 = (size_t)e->cells + 1;
+    struct Part {
+        void* buf;
+        size_t bytes;
+    };
+    const Part parts[] = {{e->x, 16 * n}, {e->v, 16 * n}, {e->prev, 16 * n}, {e->keys, 4 * n},
+                          {e->cell_start, 4 * c1}, {e->sorted_index, 4 * n}, {e->nbr, 4 * n},
+                          {e->lambdas, 4 * n}, {e->foam, 4 * n}, {e->sorted[e->final_buf], 16 * n}};
+    size_t total = 0;
+    for (const Part& q : parts) total += (q.bytes + 255) / 256 * 256;
+    struct Scratch {  // freed on every return path
+        char* p = nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        ~Scratch() {
+            if (p) (void)hipFree(p);
+            if (t0) (void)hipEventDestroy(t0);
+            if (t1) (void)hipEventDestroy(t1);
+        }
+    } sc;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&sc.p), total));
+    HIP_TRY(hipEventCreate(&sc.t0));
+    HIP_TRY(hipEventCreate(&sc.t1));
+    auto copy_parts = [&](bool save) {
+        size_t off = 0;
+        for (const Part& q : parts) {
+            void* dst = save ? sc.p + off : q.buf;
+            const void* src = save ? q.buf : sc.p + off;
+            const hipError_t r = hipMemcpyAsync(dst, src, q.bytes, hipMemcpyDeviceToDevice, s);
+            if (r != hipSuccess) return r;
+            off += (q.bytes + 255) / 256 * 256;
+        }
+        return hipSuccess;
+    };
+    const int final_buf = e->final_buf;
+    HIP_TRY(copy_parts(true));
     const bool graph = e->use_graph, prof = e->profiling;
+    const uint32_t old_block = e->stencil_block, old_pull = e->tile_pull;
     e->use_graph = false;
     e->profiling = false;
-    hipEvent_t t0, t1;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
-    uint32_t best = e->stencil_block;
-    float best_ms = 1e30f;
-    float size_ms[kNumStencilBlocks];
-    for (float& m : size_ms) m = 1e30f;
+    e->drop_graph();
+    constexpr uint32_t kConfigs = kNumStencilBlocks * kNumTilePulls;
+    float cfg_ms[kConfigs];
+    for (float& m : cfg_ms) m = 1e30f;
     mtlp_status st = MTLP_OK;
-    // two passes over the sizes, each size keeping its faster pass, so that
-    // neither the order of the sizes nor one noisy timing decides
-    for (uint32_t pass = 0; pass < 2 * kNumStencilBlocks && st == MTLP_OK; ++pass) {
-        const uint32_t bi = pass % kNumStencilBlocks;
-        const uint32_t b = kStencilBlocks[bi];
-        e->stencil_block = b;
+    // two passes over the configurations (workgroup size x tiles per pull),
+    // each keeping its faster pass, so that neither their order nor one
+    // noisy timing decides
+    for (uint32_t pass = 0; pass < 2 * kConfigs && st == MTLP_OK; ++pass) {
+        const uint32_t ci = pass % kConfigs;
+        e->stencil_block = kStencilBlocks[ci / kNumTilePulls];
+        st = mtlp_set_tile_pull(e, kTilePulls[ci % kNumTilePulls]);
         float ms = 0.0f;
         // rep 0 warms up with one step; rep 1 times kTuneSteps steps, both
-        // from the same saved state (a single step from a fresh state can
-        // favour a size that later steps do not)
+        // from the saved state (a single step from a fresh state can
+        // favour a configuration that later steps do not)
         constexpr int kTuneSteps = 8;
         for (int rep = 0; rep < 2 && st == MTLP_OK; ++rep) {
-            if (hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-                hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-                hipEventRecord(t0, s) != hipSuccess) {
+            if (hipMemcpyAsync(e->x, sc.p, 16 * n, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(e->v, sc.p + (16 * n + 255) / 256 * 256, 16 * n, hipMemcpyDeviceToDevice, s) !=
+                    hipSuccess ||
+                hipEventRecord(sc.t0, s) != hipSuccess) {
                 st = fail(MTLP_E_HIP, "autotune copy/event failed");
                 break;
             }
             for (int k = 0; k < (rep ? kTuneSteps : 1) && st == MTLP_OK; ++k) st = enqueue_step(e, s, dt);
-            if (st != MTLP_OK || hipEventRecord(t1, s) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
-                hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+            if (st != MTLP_OK || hipEventRecord(sc.t1, s) != hipSuccess || hipEventSynchronize(sc.t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, sc.t0, sc.t1) != hipSuccess) {
                 if (st == MTLP_OK) st = fail(MTLP_E_HIP, "autotune timing failed");
                 break;
             }
         }
-        if (st != MTLP_OK) break;
-        size_ms[bi] = std::min(size_ms[bi], ms);
+        if (st == MTLP_OK) cfg_ms[ci] = std::min(cfg_ms[ci], ms);
     }
-    for (uint32_t bi = 0; bi < kNumStencilBlocks; ++bi) {
-        if (size_ms[bi] < best_ms) {
-            best_ms = size_ms[bi];
-            best = kStencilBlocks[bi];
-        }
-    }
-    (void)hipMemcpyAsync(e->x, sx, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
-    (void)hipMemcpyAsync(e->v, sv, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
-    (void)hipMemcpyAsync(e->prev, sp, sizeof(float4) * e->n, hipMemcpyDeviceToDevice, s);
-    (void)hipStreamSynchronize(s);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    dev_free(sx);
-    dev_free(sv);
-    dev_free(sp);
+    uint32_t best = 0;
+    for (uint32_t ci = 1; ci < kConfigs; ++ci)
+        if (cfg_ms[ci] < cfg_ms[best]) best = ci;
+    const hipError_t restore = copy_parts(false);
+    const hipError_t sync = hipStreamSynchronize(s);
+    e->final_buf = final_buf;
     e->use_graph = graph;
     e->profiling = prof;
-    e->stencil_block = best;
+    if (st != MTLP_OK) {
+        e->stencil_block = old_block;
+        (void)mtlp_set_tile_pull(e, old_pull);
+        return st;
+    }
+    HIP_TRY(restore);
+    HIP_TRY(sync);
+    e->stencil_block = kStencilBlocks[best / kNumTilePulls];
+    MTLP_TRY(mtlp_set_tile_pull(e, kTilePulls[best % kNumTilePulls]));
     e->drop_graph();
-    if (st != MTLP_OK) return st;
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
-        out << key.str() << " " << best << "\n";
+        out << key.str() << " " << e->stencil_block << " " << e->tile_pull << "\n";
     }
-    if (block) *block = best;
+    if (block) *block = e->stencil_block;
     return MTLP_OK;
 }
 

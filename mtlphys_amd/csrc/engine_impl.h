@@ -179,6 +179,11 @@ struct mtlp_engine {
     uint32_t tile_order_tiles[2] = {0, 0};  // each holds a permutation of this many tiles (0: none)
     uint32_t tile_order_cur = 0;
     bool use_lists = true;
+    // persistent stencil launches: tiles per counter increment (0: one
+    // workgroup per tile) and the 2 x kNumXcd position counters
+    uint32_t tile_pull = 0;
+    uint32_t* tile_ctr = nullptr;
+    uint32_t cus = 256;  // compute units of the device
     // cell buffers
     uint32_t* counts = nullptr;
     uint32_t* cell_start = nullptr;
@@ -212,7 +217,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]); dev_free(tile_ctr);
         tile_order_tiles[0] = tile_order_tiles[1] = 0;
         n_alloc = 0;
     }
@@ -256,7 +261,8 @@ struct mtlp_engine {
         return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
                        heavy_count + ((it + 1u) & 1u), lanes + (size_t)(it & 1u) * lcap,
                        heavy_count + 2u + (it & 1u), heavy_count + 2u + ((it + 1u) & 1u), lcap, tile_plan,
-                       tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u]};
+                       tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u],
+                       tile_ctr, tile_pull, cus * kPersistentWavesPerCu};
     }
     // after enqueueing an iteration's passes with lists: its delta pass
     // wrote the other order

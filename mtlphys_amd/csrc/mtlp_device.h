@@ -160,10 +160,10 @@ __device__ __forceinline__ uint32_t wave_sort_by_key(uint32_t key, uint32_t val)
     return val;
 }
 
-// MI355X dispatches workgroup b to XCD b % 8, and each XCD has its own L2.
-// Renumber so that XCD x runs one contiguous run of logical blocks: slots
-// that share stencil rows then share an L2.  A bijection on [0, nb).
-constexpr uint32_t kNumXcd = 8;
+// MI355X dispatches workgroup b to XCD b % 8 (kNumXcd), and each XCD has
+// its own L2.  Renumber so that XCD x runs one contiguous run of logical
+// blocks: slots that share stencil rows then share an L2.  A bijection on
+// [0, nb).
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     const uint32_t x = b % kNumXcd, i = b / kNumXcd;
     const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;

@@ -27,6 +27,7 @@ constexpr uint32_t kLdsSortMax = 16384;   // cells up to this size: LDS bitonic 
 constexpr uint32_t kBigBlock = 1024;      // workgroup of the big-cell kernel
 constexpr uint32_t kBigGrid = 128;        // workgroups of the big-cell kernel
 constexpr uint32_t kChunkSlack = 32;      // readable slots past the end of every sorted position buffer
+constexpr uint32_t kNumXcd = 8;           // XCDs of an MI355X; workgroup b runs on XCD b % kNumXcd
 
 // Error flag bits (device word, read by mtlp_error_flags).
 constexpr uint32_t kErrScanLookback = 1u;  // a scan tile gave up waiting
@@ -199,6 +200,16 @@ struct NbrList {
     uint32_t* tile_plan;        // per tile: its cost bucket
     const uint32_t* tile_order; // both passes: tile at each position (NULL: identity)
     uint32_t* tile_order_next;  // delta pass: writes the next iteration's order here (NULL: none)
+    // Persistent launches (pull > 0): a fixed number of resident workgroups
+    // take the positions of their XCD's range `pull` tiles at a time from
+    // kTileQueues counters per XCD, instead of one workgroup per tile in
+    // dispatch order.  tile_ctr holds kTileCtrs words for the density pass
+    // (a counter every kTileCtrStride words), then kTileCtrs for the delta
+    // pass; each persistent launch zeroes the other pass's, which no launch
+    // reads until the next one of that pass.
+    uint32_t* tile_ctr = nullptr;
+    uint32_t pull = 0;
+    uint32_t pull_waves = 0;  // waves of a persistent launch (host side)
 };
 
 constexpr uint32_t kCostBuckets = 32;
@@ -209,6 +220,27 @@ constexpr uint32_t kCostSpread = 8;   // a range whose heaviest tiles are at mos
 // Workgroup sizes the stencil kernels are built for (the autotuner picks).
 constexpr uint32_t kStencilBlocks[] = {64, 128, 256, 512};
 constexpr uint32_t kNumStencilBlocks = 4;
+// Tiles per counter increment of a persistent launch (0: one workgroup per
+// tile); the autotuner tries these.
+constexpr uint32_t kTilePulls[] = {0, 1, 4};
+constexpr uint32_t kNumTilePulls = 3;
+constexpr uint32_t kMaxTilePull = 64;
+constexpr uint32_t kTileQueues = 16;     // per XCD
+constexpr uint32_t kTileCtrStride = 32;  // u32 words: one 128-byte line per counter
+constexpr uint32_t kTileCtrs = kNumXcd * kTileQueues * kTileCtrStride;  // words per pass
+// Waves a persistent stencil launch keeps resident per CU.
+constexpr uint32_t kPersistentWavesPerCu = 16;
+// Workgroups of a stencil launch of `tiles` tiles of kB threads: one per
+// tile, or (persistent) nl.pull_waves waves' worth rounded up to a multiple
+// of kNumXcd x kTileQueues, so every queue of every XCD has workgroups
+// (a queue without one would leave its positions undone).
+inline uint32_t stencil_grid(uint32_t tiles, uint32_t kB, const NbrList& nl) {
+    if (!nl.pull) return tiles;
+    constexpr uint32_t q = kNumXcd * kTileQueues;
+    const uint32_t fill = nl.pull_waves / (kB / 64u);
+    const uint32_t g = tiles < fill ? tiles : fill;
+    return (g + q - 1u) / q * q;
+}
 
 // With nl.blk non-NULL the density pass writes the list and the delta pass
 // reads it.

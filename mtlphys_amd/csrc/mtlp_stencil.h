@@ -795,6 +795,40 @@ constexpr uint32_t split_blocks() {
     return kSplitWaves / (kB / 64u);
 }
 
+// Runs tile_fn(pos) for the tile positions (of nb) this workgroup takes:
+// the one its index wg (of nwg, one per position) maps to, or, in a
+// persistent launch (kPersist, nl.pull > 0), pulls from one of the kTileQueues
+// queues of its XCD (wg % kNumXcd): queue j of XCD x hands out the
+// positions lo_x + j, lo_x + j + kTileQueues, ... of the XCD's range in
+// turn (so each queue keeps the range's heaviest-first order), nl.pull at
+// a time, from a counter of its own cache line (a single counter per XCD
+// serialised ~2,000 device-scope atomics on one address).  Every thread of
+// the workgroup must call it.
+template <bool kPersist, class TileFn>
+__device__ __forceinline__ void for_each_tile_position(const NbrList& nl, uint32_t* ctr, uint32_t wg,
+                                                       uint32_t nwg, uint32_t nb, TileFn&& tile_fn) {
+    if constexpr (!kPersist) {
+        tile_fn(xcd_block(wg, nwg));
+        return;
+    }
+    __shared__ uint32_t sh_k;
+    const uint32_t x = wg % kNumXcd, j = (wg / kNumXcd) % kTileQueues;
+    const uint32_t lo = xcd_range_start(x, nb), hi = xcd_range_start(x + 1u, nb);
+    uint32_t* c = ctr + (x * kTileQueues + j) * kTileCtrStride;
+    for (;;) {
+        if (threadIdx.x == 0) sh_k = nl.pull * atomicAdd(c, 1u);
+        __syncthreads();
+        const uint32_t k0 = sh_k;
+        __syncthreads();
+        if (lo + j + k0 * kTileQueues >= hi) break;
+        for (uint32_t u = 0; u < nl.pull; ++u) {
+            const uint32_t pos = lo + j + (k0 + u) * kTileQueues;
+            if (pos >= hi) break;
+            tile_fn(pos);
+        }
+    }
+}
+
 // Times a stencil workgroup's tile (its first wave, from the start to the
 // exit of thread 0) into a cost bucket, for the order of the next
 // iteration's passes (NbrList::tile_order).
@@ -855,7 +889,7 @@ __device__ __forceinline__ void order_range(const NbrList& nl, uint32_t x, uint3
     }
 }
 
-template <uint32_t kB, bool kTape>
+template <uint32_t kB, bool kTape, bool kPersist = false>
 __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, const StepArgs& a,
                                                   float4* __restrict__ next, float4* __restrict__ rr,
@@ -885,19 +919,23 @@ __device__ __forceinline__ void delta_launch_body(const float4* __restrict__ cur
         return;
     }
     const uint32_t ob = nl.tile_order_next ? kNumXcd : 0u;
-    const uint32_t nbt = gridDim.x - hb - lb - ob;
+    const uint32_t nbt = (a.n + kB - 1u) / kB;
     if (blockIdx.x < hb + lb + ob) {
         order_range(nl, blockIdx.x - hb - lb, nbt);
         return;
     }
-    const uint32_t pos = xcd_block(blockIdx.x - hb - lb - ob, nbt);
-    // the tiles in the density pass's order (heaviest first), else in place
-    const uint32_t tile =
-        nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
-    const uint32_t s = tile * kB + threadIdx.x;
-    if (!delta_takes_slot(s, a.n, nl)) return;
-    with_scene(s, a, [&](const SceneDev& sc) {
-        apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
+    const uint32_t wg = blockIdx.x - hb - lb - ob;
+    if (kPersist && wg == 0u)  // the density pass's counters
+        for (uint32_t q = threadIdx.x; q < kNumXcd * kTileQueues; q += kB) nl.tile_ctr[q * kTileCtrStride] = 0u;
+    for_each_tile_position<kPersist>(nl, nl.tile_ctr + kTileCtrs, wg, gridDim.x - hb - lb - ob, nbt, [&](uint32_t pos) {
+        // the tiles in the density pass's order (heaviest first), else in place
+        const uint32_t tile =
+            nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nbt - 1u) : pos;
+        const uint32_t s = tile * kB + threadIdx.x;
+        if (!delta_takes_slot(s, a.n, nl)) return;
+        with_scene(s, a, [&](const SceneDev& sc) {
+            apply_delta_slot<kTape, true>(s, sc, cur, cell_start, a.n, next, rr, nl, a.sphere);
+        });
     });
 }
 

@@ -13,48 +13,53 @@ namespace mtlp {
 
 namespace {
 
-template <uint32_t kB, bool kOrig, bool kCount, bool kList>
+template <uint32_t kB, bool kOrig, bool kCount, bool kList, bool kPersist>
 __global__ __launch_bounds__(kB) void k_density_lambda(
     float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
     uint32_t* __restrict__ nbr_orig, NbrList nl) {
     __shared__ uint2 ent[kList ? kLdsChunks * kB : 1];  // each thread's recorded chunks
-    const uint32_t nb = gridDim.x;
-    const uint32_t pos = xcd_block(blockIdx.x, nb);
-    // the tile (kB slots) this workgroup takes: in the order the last delta
-    // pass left (heaviest first within each XCD's range), else in place
-    const uint32_t tile =
-        kList && nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nb - 1u) : pos;
-    const TileTimer timer(kList ? nl : NbrList{}, tile);
-    const uint32_t s0 = tile * kB + threadIdx.x;
     const uint32_t n = a.n;
-    // The wave's slots are handed to its lanes in order of their current
-    // cell, so lanes that walk the same stencil rows sit together: their
-    // candidate loads then touch fewer cache lines per instruction.  Only
-    // the lane assignment changes.  (Full waves inside one scene.)
-    uint32_t s = s0;
-    const uint32_t wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)(s0 - __lane_id()));
-    if (wbase + 64u <= n) {
-        const uint32_t sid = scene_of(wbase, a);
-        if (sid == scene_of(wbase + 63u, a)) {
-            const SceneDev sc = load_scene_uniform(a.scenes, sid);
-            const float4 p = cur[s0];
-            s = wave_sort_by_key(key_of(cell_of(p.x, p.y, p.z, sc.g), sc.g), s0);
+    const uint32_t nb = (n + kB - 1u) / kB;  // tiles
+    if (kPersist && blockIdx.x == 0)  // the delta pass's counters
+        for (uint32_t q = threadIdx.x; q < kNumXcd * kTileQueues; q += kB) nl.tile_ctr[kTileCtrs + q * kTileCtrStride] = 0u;
+    auto tile_body = [&](uint32_t pos) {
+        // the tile (kB slots) at this position: in the order the last delta
+        // pass left (heaviest first within each XCD's range), else in place
+        const uint32_t tile =
+            kList && nl.tile_order ? min((uint32_t)__builtin_amdgcn_readfirstlane((int)nl.tile_order[pos]), nb - 1u)
+                                   : pos;
+        const TileTimer timer(kList ? nl : NbrList{}, tile);
+        const uint32_t s0 = tile * kB + threadIdx.x;
+        // The wave's slots are handed to its lanes in order of their current
+        // cell, so lanes that walk the same stencil rows sit together: their
+        // candidate loads then touch fewer cache lines per instruction.  Only
+        // the lane assignment changes.  (Full waves inside one scene.)
+        uint32_t s = s0;
+        const uint32_t wbase = (uint32_t)__builtin_amdgcn_readfirstlane((int)(s0 - __lane_id()));
+        if (wbase + 64u <= n) {
+            const uint32_t sid = scene_of(wbase, a);
+            if (sid == scene_of(wbase + 63u, a)) {
+                const SceneDev sc = load_scene_uniform(a.scenes, sid);
+                const float4 p = cur[s0];
+                s = wave_sort_by_key(key_of(cell_of(p.x, p.y, p.z, sc.g), sc.g), s0);
+            }
         }
-    }
-    if (s >= n) return;
-    with_scene(s, a, [&](const SceneDev& sc) {
-        density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
-                                                         nbr_orig, nullptr, nl, ent + threadIdx.x, kB);
-    });
+        if (s >= n) return;
+        with_scene(s, a, [&](const SceneDev& sc) {
+            density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
+                                                             nbr_orig, nullptr, nl, ent + threadIdx.x, kB);
+        });
+    };
+    for_each_tile_position<kPersist>(nl, nl.tile_ctr, blockIdx.x, gridDim.x, nb, tile_body);
 }
 
-template <uint32_t kB, bool kList>
+template <uint32_t kB, bool kList, bool kPersist>
 __global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ cur,
                                                    const uint32_t* __restrict__ cell_start,
                                                    StepArgs a, float4* __restrict__ next, NbrList nl) {
     if (kList) {
-        delta_launch_body<kB, false>(cur, cell_start, a, next, nullptr, nl);
+        delta_launch_body<kB, false, kPersist>(cur, cell_start, a, next, nullptr, nl);
         return;
     }
     const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
@@ -139,39 +144,42 @@ __global__ __launch_bounds__(kBlock) void k_mouse_pulse(const float4* __restrict
     }
 }
 
-template <uint32_t kB, bool kList>
+template <uint32_t kB, bool kList, bool kPersist>
 void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a,
                       float* lo, uint32_t* nb, const NbrList& nl, hipStream_t s) {
-    const dim3 grid(blocks_for(a.n, kB)), block(kB);
+    const dim3 grid(stencil_grid(blocks_for(a.n, kB), kB, nl)), block(kB);
     if (lo && nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, true, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb, nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, true, kList, kPersist>), grid, block, 0, s, cur, cs, sidx, a,
+                           lo, nb, nl);
     else if (lo)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, false, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
-                           nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, false, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
+                           a, lo, nb, nl);
     else if (nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, false, true, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
-                           nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, true, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
+                           a, lo, nb, nl);
     else
-        hipLaunchKernelGGL((k_density_lambda<kB, false, false, kList>), grid, block, 0, s, cur, cs, sidx, a, lo, nb,
-                           nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, false, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
+                           a, lo, nb, nl);
 }
 
 template <uint32_t kB>
 void density_lambda_bl(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float* lo,
                        uint32_t* nb, const NbrList& nl, hipStream_t s) {
-    if (nl.blk) density_lambda_b<kB, true>(cur, cs, sidx, a, lo, nb, nl, s);
-    else density_lambda_b<kB, false>(cur, cs, sidx, a, lo, nb, nl, s);
+    if (!nl.blk) density_lambda_b<kB, false, false>(cur, cs, sidx, a, lo, nb, nl, s);
+    else if (nl.pull) density_lambda_b<kB, true, true>(cur, cs, sidx, a, lo, nb, nl, s);
+    else density_lambda_b<kB, true, false>(cur, cs, sidx, a, lo, nb, nl, s);
 }
 
 template <uint32_t kB>
 void apply_delta_b(const float4* cur, const uint32_t* cs, const StepArgs& a, float4* next, const NbrList& nl,
                    hipStream_t s) {
     // with lists, the heavy queue's workgroups in front
-    const dim3 grid(blocks_for(a.n, kB) + (nl.blk ? heavy_blocks<kB>() + split_blocks<kB>() : 0u) +
-                    (nl.tile_order_next ? kNumXcd : 0u)),
+    const dim3 grid(stencil_grid(blocks_for(a.n, kB), kB, nl) +
+                    (nl.blk ? heavy_blocks<kB>() + split_blocks<kB>() : 0u) + (nl.tile_order_next ? kNumXcd : 0u)),
         block(kB);
-    if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true>), grid, block, 0, s, cur, cs, a, next, nl);
-    else hipLaunchKernelGGL((k_apply_delta<kB, false>), grid, block, 0, s, cur, cs, a, next, nl);
+    if (nl.blk && nl.pull) hipLaunchKernelGGL((k_apply_delta<kB, true, true>), grid, block, 0, s, cur, cs, a, next, nl);
+    else if (nl.blk) hipLaunchKernelGGL((k_apply_delta<kB, true, false>), grid, block, 0, s, cur, cs, a, next, nl);
+    else hipLaunchKernelGGL((k_apply_delta<kB, false, false>), grid, block, 0, s, cur, cs, a, next, nl);
 }
 
 }  // namespace

@@ -303,6 +303,17 @@ class Engine:
     def set_stencil_block(self, block):
         C.check(C.lib().mtlp_set_stencil_block(self._h, int(block)), "mtlp_set_stencil_block")
 
+    def set_tile_pull(self, tiles_per_pull):
+        """0: one workgroup per tile; k >= 1: persistent workgroups taking k
+        tiles at a time from per-XCD counters (same results)."""
+        C.check(C.lib().mtlp_set_tile_pull(self._h, int(tiles_per_pull)), "mtlp_set_tile_pull")
+
+    def launch_config(self):
+        """(stencil workgroup size, tiles per pull)."""
+        b, t = C._U32(), C._U32()
+        C.check(C.lib().mtlp_launch_config(self._h, ctypes.byref(b), ctypes.byref(t)), "mtlp_launch_config")
+        return b.value, t.value
+
     def set_neighbor_lists(self, on=True):
         """Density pass records neighbour lists for the delta pass (default on)."""
         C.check(C.lib().mtlp_set_neighbor_lists(self._h, 1 if on else 0), "mtlp_set_neighbor_lists")

@@ -132,12 +132,28 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
     with mtlp.Engine(0, p) as e:
         e.reset(20000, mtlp.capi.SCENE_EMPTY)
         e.upload_state(x, v)
-        before = e.positions_buffer().read()
+        e.step(DT)
+        before = _read_all(e)
+        before["foam"] = e.foam_intensity_buffer().read()
         blk = e.autotune(DT, cache)
         assert blk in (64, 128, 256, 512)
-        np.testing.assert_array_equal(e.positions_buffer().read(), before)  # state restored
+        cfg = e.launch_config()
+        assert cfg[0] == blk and cfg[1] in (0, 1, 4)
+        after = _read_all(e)
+        after["foam"] = e.foam_intensity_buffer().read()
+        for k in before:  # every readable buffer as before the call
+            np.testing.assert_array_equal(after[k].view(np.uint32), before[k].view(np.uint32), err_msg=k)
         assert e.autotune(DT, cache) == blk  # second call reads the cache
+        assert e.launch_config() == cfg
         assert len(cache.read_text().splitlines()) == 1
+        e.step(DT)  # the tuned configuration steps as the default one does
+        tuned = e.positions_buffer().read()
+    with mtlp.Engine(0, p) as e:
+        e.reset(20000, mtlp.capi.SCENE_EMPTY)
+        e.upload_state(x, v)
+        e.step(DT)
+        e.step(DT)
+        np.testing.assert_array_equal(e.positions_buffer().read().view(np.uint32), tuned.view(np.uint32))
         with pytest.raises(mtlp.MtlpError):
             e.set_stencil_block(100)
 
@@ -195,3 +211,62 @@ def test_ensemble_256_scenes(mtlp):
         xs = x[po[s]:po[s + 1], :3]
         assert (xs >= np.array(p.bounds_min) - 1e-6).all() and (xs <= np.array(p.bounds_max) + 1e-6).all()
     b.close()
+
+
+def test_ensemble_scenes_kernel_level_vs_fp64(mtlp, orc):
+    """Three scenes of distinct rest density and gravity taken from inside
+    the 256-scene ensemble (N_s log-uniform in [2048, 65536], 4.58M particles
+    in all, 3 iterations): every iteration's density/lambda and delta kernels
+    and the finalize, run in the batch, against the FP64 oracle fed the
+    batch's own inputs for that scene, within the single-scene tolerances
+    (lambda 1e-4 of max(|lambda|, 1e-3 max|lambda|), x 1e-5 h, v 1e-5 h/dt)."""
+    from helpers import LAMBDA_TOL, V_TOL, X_TOL_H, lambda_errors, position_error_h, velocity_error
+    from mtlphys_amd import workloads as W
+    counts, params = W.ensemble_config()
+    rho = np.array([p.rest_density for p in params])
+    grav = np.array([p.gravity[1] for p in params])
+    pick = sorted({int(np.argmin(rho)), int(np.argmax(rho)), int(np.argmin(grav))})
+    assert len(pick) == 3
+    iters = params[0].iterations
+    print("ensemble scenes", [(s, counts[s], float(rho[s]), float(grav[s])) for s in pick], flush=True)
+    runs = []
+    for k in range(iters + 1):
+        pk = [p.copy() for p in params]
+        for p in pk:
+            p.iterations = k
+        b = mtlp.Batch(counts, pk)
+        b.seed(3, 3)
+        po, co = b.offsets()
+        if k == 0:
+            x0, v0 = b.positions_buffer().read(), b.velocities_buffer().read()
+        b.step(DT)
+        got = _read_all(b)
+        assert b.error_flags() == 0
+        b.close()
+        runs.append({s: dict(sorted=got["sorted"][po[s]:po[s + 1]], lam=got["lam"][po[s]:po[s + 1]],
+                             x=got["x"][po[s]:po[s + 1]], v=got["v"][po[s]:po[s + 1]],
+                             sidx=got["sidx"][po[s]:po[s + 1]] - po[s],
+                             cs=got["cs"][co[s]:co[s + 1] + 1] - po[s]) for s in pick})
+        del got
+    for s in pick:
+        sl = slice(po[s], po[s + 1])
+        op = orc.params(params[s])
+        g = orc.grid(op)
+        h = op.h
+        cs, sidx = runs[0][s]["cs"], runs[0][s]["sidx"]
+        for k in range(iters):
+            inp = runs[k][s]["sorted"].astype(np.float64)
+            _, lam_o, _ = orc.density_lambda(inp, cs, sidx, g, op, 64)
+            le = lambda_errors(runs[k + 1][s]["lam"], lam_o)
+            nxt = orc.apply_delta(inp, runs[k + 1][s]["lam"][sidx].astype(np.float64), cs, g, op, 64)
+            xe = position_error_h(runs[k + 1][s]["sorted"], nxt, h)
+            print(f"scene {s} iteration {k}: lambda {le['rel_floor']:.2e} x {xe['max']:.2e} h", flush=True)
+            assert le["rel_floor"] <= LAMBDA_TOL, (s, k, le)
+            assert xe["max"] <= X_TOL_H, (s, k, xe)
+        fin = runs[-1][s]
+        xo = np.empty((len(sidx), 4), np.float64)
+        xo[sidx] = fin["sorted"].astype(np.float64)
+        xr, vr = orc.finalize(xo, v0[sl], x0[sl], op, DT, 64)
+        xe, ve = position_error_h(fin["x"], xr, h), velocity_error(fin["v"], vr, h, DT)
+        print(f"scene {s} finalize: x {xe['max']:.2e} h v {ve['max']:.2e} h/dt", flush=True)
+        assert xe["max"] <= X_TOL_H and ve["max"] <= V_TOL, (s, xe, ve)

@@ -395,6 +395,47 @@ def test_tile_order_does_not_change_results(mtlp, iterations):
             np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=name)
 
 
+@pytest.mark.parametrize("case", ["tiny_300", "blobs_20k", "dam_break_1m"])
+def test_persistent_launches_bit_identical(mtlp, case):
+    """Persistent stencil launches (resident workgroups taking 1 or 4 tiles
+    at a time from one counter per XCD) give the bits of one workgroup per
+    tile, also when the mode changes between steps; on the blob scene (tiles
+    of very different cost, heavy queues, two workgroup sizes) and on the 1M
+    dam break (15,625 tiles: each persistent workgroup takes several)."""
+    if case == "tiny_300":  # fewer tiles than queues
+        n = 300
+        p, x, v = _cube_case(mtlp, n=n)
+        blocks, steps = (64, 512), 4
+    elif case == "blobs_20k":
+        n = 20000
+        p, x, v = _blob_case(mtlp, n)
+        blocks, steps = (64, 256), 4
+    else:
+        n = 1_000_000
+        p = mtlp.scene_params(4, n)
+        x, v = mtlp.generate_scene(4, n, 1, p)
+        blocks, steps = (64,), 2
+    dt = 1.0 / 120.0
+    for block in blocks:
+        outs = {}
+        for name, pulls in {"dispatch": [0] * steps, "pull 1": [1] * steps, "pull 4": [4] * steps,
+                            "switching": ([0, 4, 1, 0] if steps == 4 else [4, 0])}.items():
+            with mtlp.Engine(0, p) as e:
+                e.reset(n, mtlp.capi.SCENE_EMPTY)
+                e.upload_state(x, v)
+                e.set_stencil_block(block)
+                outs[name] = []
+                for t in pulls:
+                    e.set_tile_pull(t)
+                    e.step(dt)
+                    outs[name] += [e.positions_buffer().read(), e.lambdas_buffer().read()]
+                assert e.launch_config() == (block, pulls[-1])
+                assert e.error_flags() == 0
+        for name in outs:
+            for a, b in zip(outs["dispatch"], outs[name]):
+                np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=(case, block, name))
+
+
 def test_step_free_running_vs_fp64(mtlp, orc):
     """One full step from the same state: the GPU may differ from the FP64
     oracle by no more than a few times the oracle's own FP32 instantiation
