@@ -121,6 +121,7 @@ mtlp_status enqueue_taped_step(mtlp_engine* e, hipStream_t s, float dt, uint32_t
     b.sorted_index = const_cast<uint32_t*>(t.sorted_index);
     b.sorted = t.iterate(0);
     b.slot_of = nullptr;
+    b.xyz = nullptr;  // the taped passes read the tape's float4 iterates
     StageClock clk(e, s);
     MTLP_TRY(enqueue_hash(e, s, a, clk, b));
     const uint32_t I = A.tape_iters;
@@ -148,6 +149,7 @@ mtlp_status enqueue_plain_hash(mtlp_engine* e, hipStream_t s, const float* d_x4)
     const StepArgs a = e->step_args(0.0f, false);
     HashBufs b = e->hash_bufs();
     b.x = reinterpret_cast<const float4*>(d_x4);
+    b.xyz = nullptr;
     StageClock clk(e, s);
     MTLP_TRY(enqueue_hash(e, s, a, clk, b));
     e->final_buf = 0;

@@ -212,6 +212,12 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
     HIP_TRY(dev_alloc(&e->slot_of, na));
     HIP_TRY(dev_alloc(&e->sorted[0], na + kChunkSlack));  // the stencil filter reads past the end
     HIP_TRY(dev_alloc(&e->sorted[1], na + kChunkSlack));
+    if ((size_t)na + kChunkSlack <= kMaxXyzSlots) {
+        HIP_TRY(dev_alloc(&e->xyz[0], 3 * ((size_t)na + kChunkSlack)));
+        HIP_TRY(dev_alloc(&e->xyz[1], 3 * ((size_t)na + kChunkSlack)));
+        HIP_TRY(hipMemset(e->xyz[0], 0, 3 * sizeof(float) * (na + kChunkSlack)));
+        HIP_TRY(hipMemset(e->xyz[1], 0, 3 * sizeof(float) * (na + kChunkSlack)));
+    }
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
     HIP_TRY(dev_alloc(&e->nlist, (size_t)kListBlocks * na));
     HIP_TRY(dev_alloc(&e->ncount, na));
@@ -331,7 +337,11 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     const uint32_t iters = e->iterations();
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
-        const NbrList nl = e->nbr_list(it);
+        NbrList nl = e->nbr_list(it);
+        if (nl.blk && e->xyz[0]) {  // the filter reads this iterate's packed copy; the delta pass writes the next one
+            nl.xyz = e->xyz[cur];
+            nl.xyz_next = e->xyz[cur ^ 1];
+        }
         HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a, nullptr,
                                       it == 0 ? e->nbr : nullptr, nl, e->stencil_block, s));
         clk.mark("density_lambda");

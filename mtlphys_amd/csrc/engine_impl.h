@@ -162,6 +162,7 @@ struct mtlp_engine {
     uint32_t* sorted_index = nullptr;
     uint32_t* slot_of = nullptr;  // sorted slot of each particle
     float4* sorted[2] = {nullptr, nullptr};
+    float* xyz[2] = {nullptr, nullptr};  // packed (x, y, z) of sorted[0], sorted[1] (NbrList::xyz)
     uint32_t* big_list = nullptr;
     int final_buf = 0;
     // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
@@ -217,7 +218,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]); dev_free(tile_ctr);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(xyz[0]); dev_free(xyz[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]); dev_free(tile_ctr);
         tile_order_tiles[0] = tile_order_tiles[1] = 0;
         n_alloc = 0;
     }
@@ -246,6 +247,7 @@ struct mtlp_engine {
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;
+        b.xyz = use_lists ? xyz[0] : nullptr;
         return b;
     }
 

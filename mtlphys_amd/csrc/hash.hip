@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const uint2* __restrict__ tmp, const float4* __restrict__ stage,
     const uint32_t* __restrict__ cell_start, uint32_t n, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted, uint32_t* __restrict__ big_list, uint32_t* __restrict__ big_count,
-    uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of) {
+    uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of, float* __restrict__ xyz) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint2 e = tmp[s];
@@ -91,7 +91,9 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
     for (uint32_t m = lo; m < hi; ++m) rank += (tmp[m].x < i) ? 1u : 0u;
     const uint32_t dst = lo + rank;
     sorted_index[dst] = i;
-    sorted[dst] = stage[s];
+    const float4 p = stage[s];
+    sorted[dst] = p;
+    store_xyz(xyz, dst, p);
     if (slot_of) slot_of[i] = dst;
 }
 
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, const uint32_t* __restrict__ big_list,
     const uint32_t* __restrict__ big_count, uint32_t* __restrict__ sorted_index,
-    float4* __restrict__ sorted, uint32_t* __restrict__ slot_of) {
+    float4* __restrict__ sorted, uint32_t* __restrict__ slot_of, float* __restrict__ xyz) {
     __shared__ uint32_t lds[kLdsSortMax];
     __shared__ uint32_t s_wave[kBigBlock / 64];
     const uint32_t t = threadIdx.x;
@@ -140,7 +142,9 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
             for (uint32_t k = t; k < len; k += kBigBlock) {
                 const uint32_t i = lds[k];
                 sorted_index[lo + k] = i;
-                sorted[lo + k] = binned_pos(x, v, i, a);
+                const float4 p = binned_pos(x, v, i, a);
+                sorted[lo + k] = p;
+                store_xyz(xyz, lo + k, p);
                 if (slot_of) slot_of[i] = lo + k;
             }
             __syncthreads();
@@ -164,7 +168,9 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
                     const uint32_t dst = lo + run + pre + below;
                     if (dst < hi) {
                         sorted_index[dst] = j;
-                        sorted[dst] = binned_pos(x, v, j, a);
+                        const float4 p = binned_pos(x, v, j, a);
+                        sorted[dst] = p;
+                        store_xyz(xyz, dst, p);
                         if (slot_of) slot_of[j] = dst;
                     }
                 }
@@ -232,13 +238,13 @@ hipError_t launch_scatter_gather(const HashBufs& b, const StepArgs& a, hipStream
     hipLaunchKernelGGL(k_scatter_ticket, dim3(grid), dim3(kBlock), 0, s, b.keys, b.tickets,
                        b.cell_start, b.x, b.v, a, b.tmp, b.stage, b.err);
     hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp, b.stage, b.cell_start, a.n,
-                       b.sorted_index, b.sorted, b.big_list, b.big_count, b.err, b.slot_of);
+                       b.sorted_index, b.sorted, b.big_list, b.big_count, b.err, b.slot_of, b.xyz);
     // Only cells above kSmallCell particles exist past this point; with
     // n <= kSmallCell there can be none.
     if (a.n > kSmallCell)
         hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp, b.keys,
                            b.cell_start, b.x, b.v, a, b.big_list, b.big_count, b.sorted_index,
-                           b.sorted, b.slot_of);
+                           b.sorted, b.slot_of, b.xyz);
     return hipGetLastError();
 }
 

@@ -119,6 +119,17 @@ __device__ __forceinline__ float4 binned_pos(const float4* __restrict__ x, const
     return out;
 }
 
+// Slot s of a packed (x, y, z) position copy (12 bytes a slot; NULL: none).
+// The byte offset is 32-bit from the uniform base (one VGPR for the
+// address); copies are allocated for at most kMaxXyzSlots slots.
+__device__ __forceinline__ void store_xyz(float* __restrict__ xyz, uint32_t s, const float4& p) {
+    if (!xyz) return;
+    float* q = reinterpret_cast<float*>(reinterpret_cast<char*>(xyz) + s * 12u);
+    q[0] = p.x;
+    q[1] = p.y;
+    q[2] = p.z;
+}
+
 // Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 inside each 16-lane
 // row, then row_bcast:15 and row_bcast:31 carry row totals forward.  Lanes
 // whose DPP source is invalid or whose row is masked off read `old` = 0.

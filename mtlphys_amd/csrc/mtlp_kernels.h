@@ -27,6 +27,9 @@ constexpr uint32_t kLdsSortMax = 16384;   // cells up to this size: LDS bitonic 
 constexpr uint32_t kBigBlock = 1024;      // workgroup of the big-cell kernel
 constexpr uint32_t kBigGrid = 128;        // workgroups of the big-cell kernel
 constexpr uint32_t kChunkSlack = 32;      // readable slots past the end of every sorted position buffer
+// Largest packed (x, y, z) copy (slots incl. slack): its 12-byte slots are
+// addressed with 32-bit byte offsets.  Larger engines run without copies.
+constexpr uint32_t kMaxXyzSlots = 0xffffffffu / 12u;
 constexpr uint32_t kNumXcd = 8;           // XCDs of an MI355X; workgroup b runs on XCD b % kNumXcd
 
 // Error flag bits (device word, read by mtlp_error_flags).
@@ -109,6 +112,7 @@ struct HashBufs {
     uint32_t* err;
     uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
     uint32_t* heavy_count; // 4 heavy-queue counters, cleared by the build; may be NULL
+    float* xyz = nullptr;  // packed (x, y, z) of `sorted`, 12 bytes a slot; may be NULL
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------
@@ -210,6 +214,14 @@ struct NbrList {
     uint32_t* tile_ctr = nullptr;
     uint32_t pull = 0;
     uint32_t pull_waves = 0;  // waves of a persistent launch (host side)
+    // Packed (x, y, z) copies of the iterates, 12 bytes a slot with
+    // kChunkSlack slots of read slack: the density pass's stencil filter
+    // reads its candidates from xyz (eight of them in six 16-byte loads
+    // instead of eight 12-byte ones), and the delta pass writes the next
+    // iterate's copy to xyz_next beside it.  NULL: the filter reads the
+    // float4 iterate.
+    const float* xyz = nullptr;
+    float* xyz_next = nullptr;
 };
 
 constexpr uint32_t kCostBuckets = 32;

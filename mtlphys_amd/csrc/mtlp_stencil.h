@@ -31,6 +31,16 @@ __device__ __forceinline__ float dist2(float rx, float ry, float rz) {
     return rx * rx + ry * ry + rz * rz;
 }
 
+// The chunk's mask from the sign bits shifted in by a filter's groups of
+// eight: candidate k went in at bit 8 * groups - 1 - k; bits past the
+// chunk's len candidates and the bit of s itself are cleared.
+__device__ __forceinline__ uint32_t chunk_mask(uint32_t shifted, uint32_t c0, uint32_t len, uint32_t s) {
+    uint32_t mask = __builtin_bitreverse32(shifted) >> (32u - 8u * ((min(len, 32u) + 7u) / 8u));
+    mask &= len >= 32u ? 0xffffffffu : (1u << len) - 1u;
+    const uint32_t self = s - c0;  // wraps to a large value when s < c0
+    return self < 32u ? mask & ~(1u << self) : mask;
+}
+
 // Candidates are filtered before any SPH arithmetic: bit k of the result is
 // set when slot c0 + k (k < c1 - c0 <= 32) is a neighbour of s, i.e.
 // |p_s - p_j|^2 < h^2 and j != s.  The expensive terms then run only on set
@@ -56,11 +66,42 @@ __device__ __forceinline__ uint32_t neighbor_mask(const float4* __restrict__ cur
             mask = __builtin_amdgcn_alignbit(mask, __float_as_uint(d), 31);
         }
     }
-    // candidate k went in at bit 8 * trips - 1 - k
-    mask = __builtin_bitreverse32(mask) >> (32u - 8u * ((min(len, 32u) + 7u) / 8u));
-    mask &= len >= 32u ? 0xffffffffu : (1u << len) - 1u;
-    const uint32_t self = s - c0;  // wraps to a large value when s < c0
-    return self < 32u ? mask & ~(1u << self) : mask;
+    return chunk_mask(mask, c0, len, s);
+}
+
+// neighbor_mask over a packed (x, y, z) copy of the iterate: the eight
+// candidates of a group are 96 contiguous bytes, fetched as six 16-byte
+// loads (4-byte aligned) instead of eight 12-byte ones, which is the
+// filter's share of the vector-memory instructions that bound the density
+// pass.  Same values, same arithmetic, same bits as neighbor_mask.  Reads
+// up to kChunkSlack slots past the last particle, like neighbor_mask.
+struct __attribute__((packed, aligned(4))) Words4 {
+    float a, b, c, d;
+};
+__device__ __forceinline__ uint32_t neighbor_mask_xyz(const float* __restrict__ xyz, uint32_t c0, uint32_t c1,
+                                                      uint32_t s, float4 pi, float h2) {
+    const float* q = xyz + 3 * (size_t)c0;
+    const uint32_t len = c1 - c0;  // >= 1
+    uint32_t mask = 0;
+#pragma unroll
+    for (uint32_t k0 = 0; k0 < 32; k0 += 8) {
+        if (k0 >= len) break;
+        float t[24];
+#pragma unroll
+        for (uint32_t u = 0; u < 6; ++u) {
+            const Words4 w = reinterpret_cast<const Words4*>(q + 3 * k0)[u];
+            t[4 * u] = w.a;
+            t[4 * u + 1] = w.b;
+            t[4 * u + 2] = w.c;
+            t[4 * u + 3] = w.d;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const float d = nbr_key(pi.x - t[3 * u], pi.y - t[3 * u + 1], pi.z - t[3 * u + 2], h2);
+            mask = __builtin_amdgcn_alignbit(mask, __float_as_uint(d), 31);
+        }
+    }
+    return chunk_mask(mask, c0, len, s);
 }
 
 // Slot ranges of the 9 stencil rows (z-major, then y), fetched together.
@@ -108,20 +149,26 @@ __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* 
 // With kUnrollRows the 9-row loop is unrolled, so the rows' slot ranges
 // stay in registers: indexed by a loop counter they go to scratch memory
 // when LDS holds other data (the density pass's chunk columns), 18 extra
-// vector-memory reads per slot.
+// vector-memory reads per slot; the candidates then come from xyz, the
+// iterate's packed copy, when one is given.
 template <bool kUnrollRows = false, class Body>
 __device__ __forceinline__ void for_each_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
                                                const float4* __restrict__ cur,
                                                const uint32_t* __restrict__ cell_start, uint32_t n,
-                                               Body&& body) {
+                                               Body&& body, const float* __restrict__ xyz = nullptr) {
     const Stencil st = stencil_of(pi.x, pi.y, pi.z, sc.g);
     const Rows rows = stencil_rows(st, cell_start, sc.g, sc.cell_off, n);
     if constexpr (kUnrollRows) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const uint32_t b = rows.b[k];
-            for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
-                body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+            if (xyz) {  // uniform: one kernel argument
+                for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
+                    body(c0, neighbor_mask_xyz(xyz, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+            } else {
+                for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
+                    body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+            }
         }
     } else {
 #pragma unroll 1
@@ -185,7 +232,8 @@ __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& p
 // Walks slot s's stencil, keeping its first kLdsChunks chunks (c0, mask)
 // in this lane's LDS column le (stride lst) and chunks kLdsChunks ..
 // kChunkScratch - 1 at their index in nl.spill; returns the number of
-// chunks and the number of neighbours in nbrs.
+// chunks and the number of neighbours in nbrs.  The candidates come from
+// nl.xyz when the iterate has a packed copy there.
 __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
@@ -199,7 +247,7 @@ __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, 
         if (K - kLdsChunks < kChunkScratch - kLdsChunks) nl.spill[(size_t)K * nl.stride + s] = e;
         ++K;
         c += (uint32_t)__builtin_popcount(mask);
-    });
+    }, nl.xyz);
     nbrs = c;
     return K;
 }
@@ -546,12 +594,14 @@ __device__ __forceinline__ float4 sphere_push(float4 p, const SphereDev& sp) {
 
 // The end of applyDeltaSorted for slot s from R = the sum of its terms:
 // next[s] = clamp(p_s + cap(R / rho0)), then the sphere-drop scene's
-// sphereCollision (Engine.cpp:760-771).  With kTape R / rho0 (before the
-// 0.3 h cap) goes to rr[s] for the adjoint.
+// sphereCollision (Engine.cpp:760-771); its packed copy goes to xyz_next
+// when there is one.  With kTape R / rho0 (before the 0.3 h cap) goes to
+// rr[s] for the adjoint.
 template <bool kTape>
 __device__ __forceinline__ void finish_delta(uint32_t s, const float4& pi, const SolverDev& sd, double dx,
                                              double dy, double dz, float4* __restrict__ next,
-                                             float4* __restrict__ rr, const SphereDev& sphere) {
+                                             float4* __restrict__ rr, const SphereDev& sphere,
+                                             float* __restrict__ xyz_next) {
 #pragma clang fp contract(off)
     dx *= sd.inv_rho0_d;
     dy *= sd.inv_rho0_d;
@@ -570,6 +620,7 @@ __device__ __forceinline__ void finish_delta(uint32_t s, const float4& pi, const
                            (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
     if (sphere.on) q = sphere_push(q, sphere);
     next[s] = q;
+    store_xyz(xyz_next, s, q);
 }
 
 // applyDeltaSorted for slot s (Constraints.metal:140-199): R_s = sum_j
@@ -589,7 +640,7 @@ __device__ __forceinline__ void apply_delta_slot(uint32_t s, const SceneDev& sc,
         s, pi, sc, cur, cell_start, n, nl, [&](uint32_t j) { return cur[j]; },
         [&](uint32_t, const float4& q) { r.add(delta_coef(pi, lam_d, q, sd)); });
     r.finish();
-    finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere);
+    finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere, nl.xyz_next);
 }
 
 __device__ __forceinline__ double readlane_d(double v, int lane) {
@@ -653,7 +704,7 @@ __device__ __forceinline__ void apply_delta_heavy(uint32_t s, const SceneDev& sc
         }
     }
     r.finish();
-    if (lane == 0) finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere);
+    if (lane == 0) finish_delta<kTape>(s, pi, sd, r.dx, r.dy, r.dz, next, rr, sphere, nl.xyz_next);
 }
 
 // The walk of block B (neighbours kSumBlock B .. kSumBlock (B + 1) - 1 in
@@ -772,7 +823,7 @@ __device__ __forceinline__ void apply_delta_split(uint32_t s, bool q_ok, const S
         }
     }
     if (q_ok && b == 0u) {
-        with_scene(s, a, [&](const SceneDev& sc) { finish_delta<kTape>(s, pi, sc.sd, dx, dy, dz, next, rr, a.sphere); });
+        with_scene(s, a, [&](const SceneDev& sc) { finish_delta<kTape>(s, pi, sc.sd, dx, dy, dz, next, rr, a.sphere, nl.xyz_next); });
     }
 }
 

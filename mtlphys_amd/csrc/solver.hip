@@ -54,8 +54,11 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
     for_each_tile_position<kPersist>(nl, nl.tile_ctr, blockIdx.x, gridDim.x, nb, tile_body);
 }
 
+// Five waves per SIMD: the list walk is latency-bound, and the packed-copy
+// store of finish_delta would otherwise take the register count one past the
+// 96 that allow five.
 template <uint32_t kB, bool kList, bool kPersist>
-__global__ __launch_bounds__(kB) void k_apply_delta(const float4* __restrict__ cur,
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(5))) void k_apply_delta(const float4* __restrict__ cur,
                                                    const uint32_t* __restrict__ cell_start,
                                                    StepArgs a, float4* __restrict__ next, NbrList nl) {
     if (kList) {
