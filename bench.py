@@ -177,7 +177,7 @@ def batched_leg(steps, warmup, stream, torch):
     b = W.make_ensemble()
     n = b.particle_count
     block = b.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"))
-    pull = b.launch_config()[1]
+    _, pull, packed = b.launch_config()
     b.set_graph(True)
     for _ in range(warmup):
         b.step(DT, stream)
@@ -207,7 +207,7 @@ def batched_leg(steps, warmup, stream, torch):
     bytes_single = W.step_bytes(n, single.total_cells, p.iterations)
     single.close()
     gbs_b, gbs_s = bytes_batch / t_batch / 1e9, bytes_single / t_single / 1e9
-    return {"scenes": 256, "particles": n, "iterations": iters, "stencil_block": block, "tile_pull": pull,
+    return {"scenes": 256, "particles": n, "iterations": iters, "stencil_block": block, "tile_pull": pull, "packed_positions": packed,
             "ms_per_step": round(t_batch * 1e3, 4), "value": round(gbs_b, 2), "unit": "GB/s",
             "single_scene_ms_per_step": round(t_single * 1e3, 4), "single_scene_gbs": round(gbs_s, 2),
             "ratio_to_single_scene": round(gbs_b / gbs_s, 3), "error_flags": flags,
@@ -506,7 +506,7 @@ def main():
     # warmed-up state (the engine's autotuner times a few steps per size and
     # restores the state)
     block = eng.autotune(DT, os.path.join(ROOT, "mtlphys_amd", "autotune_cache.txt"), stream)
-    pull = eng.launch_config()[1]
+    _, pull, packed = eng.launch_config()
     warm_state = (params, eng.positions_buffer().read(), eng.velocities_buffer().read())
 
     # timed region: K steps, HIP events after every kernel on the launch stream
@@ -596,7 +596,7 @@ def main():
         "data": "synthetic: PCG32-jittered lattice dam break, seed 1",
         "config": {"workload": "dam_break_1m", "particles": N_PARTICLES, "h": params.h, "cells": cells,
                    "grid": list(mtlp.grid_dims(params)), "iterations": iters, "dt": DT, "stencil_block": block,
-                   "tile_pull": pull, "alg_bytes_per_step": b_step,
+                   "tile_pull": pull, "packed_positions": packed, "alg_bytes_per_step": b_step,
                    "stored_bytes_per_step": step_bytes_stored(N_PARTICLES, cells, iters)},
         "roofline": roofline,
         "roofline_valu": roofline_valu,

@@ -250,8 +250,9 @@ mtlp_status mtlp_step_n(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t n
 mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
 /* Times eight steps at each launch configuration of the stencil kernels
  * (workgroup size 64, 128, 256 or 512 times tiles per pull 0, 1 or 4, see
- * mtlp_set_tile_pull), two passes over them, from the current state, and
- * keeps the configuration with the fastest pass.  Afterwards every buffer a
+ * mtlp_set_tile_pull, times packed positions on or off, see
+ * mtlp_set_packed_positions), two passes over them, from the current state,
+ * and keeps the configuration with the fastest pass.  Afterwards every buffer a
  * caller can read (positions, velocities, previous positions, cell hashes,
  * cell offsets, sorted index and positions, neighbour counts, lambdas,
  * foam) holds what it held before the call.  With cache_path, reads a
@@ -268,9 +269,19 @@ mtlp_status mtlp_set_stencil_block(mtlp_engine* e, uint32_t block);
  * working through its own range of the tile order.  Results are the same
  * bits either way. */
 mtlp_status mtlp_set_tile_pull(mtlp_engine* e, uint32_t tiles_per_pull);
-/* The stencil kernels' current workgroup size and tiles per pull (either
- * pointer may be NULL). */
-mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull);
+/* Packed positions (no reference counterpart; on by default): with
+ * neighbour lists on, each iterate also gets a packed 12-byte (x, y, z)
+ * copy, written by the spatial hash's gather and by the delta pass, and the
+ * density pass filters its stencil candidates from it (eight candidates in
+ * six 16-byte loads instead of eight 12-byte ones).  Faster where the
+ * stencil filter dominates (the 1M dam break); the autotuner tries both.
+ * Results are the same bits either way.  MTLP_E_RANGE when enabling it on
+ * an engine of more than 357,913,941 slots, which has no copies. */
+mtlp_status mtlp_set_packed_positions(mtlp_engine* e, int enable);
+/* The stencil kernels' current workgroup size, tiles per pull and packed
+ * positions setting (any pointer may be NULL). */
+mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull,
+                               int* packed_positions);
 /* Neighbour lists (on by default): each iteration's density pass records
  * up to 96 neighbours per particle, or the stencil chunks that hold them
  * (644 B of device memory per particle), and the delta pass walks them

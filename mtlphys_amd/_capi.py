@@ -102,7 +102,8 @@ SIGNATURES = {
     "mtlp_exclusive_scan": (_I, [_P, _P, _P, _P, _P, _U32]),
     "mtlp_exclusive_scan_noalloc": (_I, [_P, _P, _P, _U32]),
     "mtlp_set_tile_pull": (_I, [_P, _U32]),
-    "mtlp_launch_config": (_I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32)]),
+    "mtlp_set_packed_positions": (_I, [_P, ctypes.c_int]),
+    "mtlp_launch_config": (_I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U32), ctypes.POINTER(ctypes.c_int)]),
     "mtlp_get_buffer": (_I, [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_size_t)]),
     "mtlp_read_buffer": (_I, [_P, _I, _P, ctypes.c_size_t]),
     "mtlp_write_buffer": (_I, [_P, _I, _P, ctypes.c_size_t]),

@@ -337,11 +337,7 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
     const uint32_t iters = e->iterations();
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {  // Engine.cpp:732-774
-        NbrList nl = e->nbr_list(it);
-        if (nl.blk && e->xyz[0]) {  // the filter reads this iterate's packed copy; the delta pass writes the next one
-            nl.xyz = e->xyz[cur];
-            nl.xyz_next = e->xyz[cur ^ 1];
-        }
+        const NbrList nl = e->nbr_list(it);
         HIP_TRY(launch_density_lambda(e->sorted[cur], e->cell_start, e->sorted_index, a, nullptr,
                                       it == 0 ? e->nbr : nullptr, nl, e->stencil_block, s));
         clk.mark("density_lambda");
@@ -749,10 +745,23 @@ mtlp_status mtlp_set_tile_pull(mtlp_engine* e, uint32_t tiles_per_pull) {
     return MTLP_OK;
 }
 
-mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull) {
+mtlp_status mtlp_set_packed_positions(mtlp_engine* e, int enable) {
+    if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
+    if (enable && e->allocated && !e->xyz[0])
+        return fail(MTLP_E_RANGE, "engine too large for packed position copies");
+    // every step's hash writes iteration 0's copy and each delta pass the
+    // next one before a density pass reads it, so switching needs no sync
+    if (e->packed_positions != (enable != 0)) e->drop_graph();
+    e->packed_positions = enable != 0;
+    return MTLP_OK;
+}
+
+mtlp_status mtlp_launch_config(const mtlp_engine* e, uint32_t* block, uint32_t* tiles_per_pull,
+                               int* packed_positions) {
     if (!e) return fail(MTLP_E_INVALID, "engine is NULL");
     if (block) *block = e->stencil_block;
     if (tiles_per_pull) *tiles_per_pull = e->tile_pull;
+    if (packed_positions) *packed_positions = e->packed_positions && (!e->allocated || e->xyz[0]) ? 1 : 0;
     return MTLP_OK;
 }
 
@@ -769,9 +778,9 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
         while (std::getline(in, line)) {
             std::istringstream ls(line);
             std::string k;
-            uint32_t b = 0, pull = 0;
-            if (ls >> k >> b >> pull && k == key.str() && mtlp_set_stencil_block(e, b) == MTLP_OK &&
-                mtlp_set_tile_pull(e, pull) == MTLP_OK) {
+            uint32_t b = 0, pull = 0, packed = 0;
+            if (ls >> k >> b >> pull >> packed && k == key.str() && mtlp_set_stencil_block(e, b) == MTLP_OK &&
+                mtlp_set_tile_pull(e, pull) == MTLP_OK && mtlp_set_packed_positions(e, (int)packed) == MTLP_OK) {
                 if (block) *block = b;
                 return MTLP_OK;
             }
@@ -824,20 +833,24 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
     HIP_TRY(copy_parts(true));
     const bool graph = e->use_graph, prof = e->profiling;
     const uint32_t old_block = e->stencil_block, old_pull = e->tile_pull;
+    const bool old_packed = e->packed_positions;
+    const uint32_t packs = e->xyz[0] ? 2u : 1u;  // packed positions on, off (only off without copies)
     e->use_graph = false;
     e->profiling = false;
     e->drop_graph();
-    constexpr uint32_t kConfigs = kNumStencilBlocks * kNumTilePulls;
-    float cfg_ms[kConfigs];
+    constexpr uint32_t kLaunches = kNumStencilBlocks * kNumTilePulls;
+    const uint32_t configs = kLaunches * packs;
+    float cfg_ms[2 * kLaunches];
     for (float& m : cfg_ms) m = 1e30f;
     mtlp_status st = MTLP_OK;
-    // two passes over the configurations (workgroup size x tiles per pull),
-    // each keeping its faster pass, so that neither their order nor one
-    // noisy timing decides
-    for (uint32_t pass = 0; pass < 2 * kConfigs && st == MTLP_OK; ++pass) {
-        const uint32_t ci = pass % kConfigs;
-        e->stencil_block = kStencilBlocks[ci / kNumTilePulls];
-        st = mtlp_set_tile_pull(e, kTilePulls[ci % kNumTilePulls]);
+    // two passes over the configurations (workgroup size x tiles per pull x
+    // packed positions), each keeping its faster pass, so that neither their
+    // order nor one noisy timing decides
+    for (uint32_t pass = 0; pass < 2 * configs && st == MTLP_OK; ++pass) {
+        const uint32_t ci = pass % configs, li = ci % kLaunches;
+        e->stencil_block = kStencilBlocks[li / kNumTilePulls];
+        e->packed_positions = packs == 2u && ci < kLaunches;
+        st = mtlp_set_tile_pull(e, kTilePulls[li % kNumTilePulls]);
         float ms = 0.0f;
         // rep 0 warms up with one step; rep 1 times kTuneSteps steps, both
         // from the saved state (a single step from a fresh state can
@@ -861,7 +874,7 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
         if (st == MTLP_OK) cfg_ms[ci] = std::min(cfg_ms[ci], ms);
     }
     uint32_t best = 0;
-    for (uint32_t ci = 1; ci < kConfigs; ++ci)
+    for (uint32_t ci = 1; ci < configs; ++ci)
         if (cfg_ms[ci] < cfg_ms[best]) best = ci;
     const hipError_t restore = copy_parts(false);
     const hipError_t sync = hipStreamSynchronize(s);
@@ -870,17 +883,20 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
     e->profiling = prof;
     if (st != MTLP_OK) {
         e->stencil_block = old_block;
+        e->packed_positions = old_packed;
         (void)mtlp_set_tile_pull(e, old_pull);
         return st;
     }
     HIP_TRY(restore);
     HIP_TRY(sync);
-    e->stencil_block = kStencilBlocks[best / kNumTilePulls];
-    MTLP_TRY(mtlp_set_tile_pull(e, kTilePulls[best % kNumTilePulls]));
+    e->stencil_block = kStencilBlocks[(best % kLaunches) / kNumTilePulls];
+    e->packed_positions = packs == 2u && best < kLaunches;
+    MTLP_TRY(mtlp_set_tile_pull(e, kTilePulls[(best % kLaunches) % kNumTilePulls]));
     e->drop_graph();
     if (cache_path) {
         std::ofstream out(cache_path, std::ios::app);
-        out << key.str() << " " << e->stencil_block << " " << e->tile_pull << "\n";
+        out << key.str() << " " << e->stencil_block << " " << e->tile_pull << " " << (e->packed_positions ? 1 : 0)
+            << "\n";
     }
     if (block) *block = e->stencil_block;
     return MTLP_OK;

@@ -163,6 +163,7 @@ struct mtlp_engine {
     uint32_t* slot_of = nullptr;  // sorted slot of each particle
     float4* sorted[2] = {nullptr, nullptr};
     float* xyz[2] = {nullptr, nullptr};  // packed (x, y, z) of sorted[0], sorted[1] (NbrList::xyz)
+    bool packed_positions = true;        // filter from xyz (mtlp_set_packed_positions); off without copies
     uint32_t* big_list = nullptr;
     int final_buf = 0;
     // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
@@ -247,7 +248,7 @@ struct mtlp_engine {
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;
-        b.xyz = use_lists ? xyz[0] : nullptr;
+        b.xyz = use_lists && packed_positions ? xyz[0] : nullptr;  // iteration 0's copy (may be NULL)
         return b;
     }
 
@@ -256,15 +257,23 @@ struct mtlp_engine {
     // lists of solver iteration `it` (its heavy queue by parity); the
     // passes follow tile_order[tile_order_cur] when it is a permutation of
     // this launch's tiles
+    // Iteration `it` reads iterate sorted[it & 1] and writes the other one;
+    // with packed positions on, its density pass filters from that
+    // iterate's packed copy and its delta pass writes the next one.
     NbrList nbr_list(uint32_t it) const {
         if (!use_lists) return NbrList{};
         const uint32_t c = tile_order_cur;
         const uint32_t lcap = lanes_cap_for(n_alloc);
-        return NbrList{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
-                       heavy_count + ((it + 1u) & 1u), lanes + (size_t)(it & 1u) * lcap,
-                       heavy_count + 2u + (it & 1u), heavy_count + 2u + ((it + 1u) & 1u), lcap, tile_plan,
-                       tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u],
-                       tile_ctr, tile_pull, cus * kPersistentWavesPerCu};
+        NbrList nl{nlist, ncount, n_alloc, nspill, heavy + (it & 1u) * kHeavyCap, heavy_count + (it & 1u),
+                   heavy_count + ((it + 1u) & 1u), lanes + (size_t)(it & 1u) * lcap,
+                   heavy_count + 2u + (it & 1u), heavy_count + 2u + ((it + 1u) & 1u), lcap, tile_plan,
+                   tile_order_tiles[c] == stencil_tiles() ? tile_order[c] : nullptr, tile_order[c ^ 1u],
+                   tile_ctr, tile_pull, cus * kPersistentWavesPerCu};
+        if (packed_positions && xyz[0]) {
+            nl.xyz = xyz[it & 1u];
+            nl.xyz_next = xyz[(it + 1u) & 1u];
+        }
+        return nl;
     }
     // after enqueueing an iteration's passes with lists: its delta pass
     // wrote the other order

@@ -149,9 +149,10 @@ __device__ __forceinline__ Rows stencil_rows(const Stencil& st, const uint32_t* 
 // With kUnrollRows the 9-row loop is unrolled, so the rows' slot ranges
 // stay in registers: indexed by a loop counter they go to scratch memory
 // when LDS holds other data (the density pass's chunk columns), 18 extra
-// vector-memory reads per slot; the candidates then come from xyz, the
-// iterate's packed copy, when one is given.
-template <bool kUnrollRows = false, class Body>
+// vector-memory reads per slot.  With kXyz the candidates come from xyz,
+// the iterate's packed copy (a template choice, so each kernel carries one
+// filter: both in one kernel measured slower on the ensemble's later steps).
+template <bool kUnrollRows = false, bool kXyz = false, class Body>
 __device__ __forceinline__ void for_each_chunk(uint32_t s, const float4& pi, const SceneDev& sc,
                                                const float4* __restrict__ cur,
                                                const uint32_t* __restrict__ cell_start, uint32_t n,
@@ -162,12 +163,10 @@ __device__ __forceinline__ void for_each_chunk(uint32_t s, const float4& pi, con
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             const uint32_t b = rows.b[k];
-            if (xyz) {  // uniform: one kernel argument
-                for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
-                    body(c0, neighbor_mask_xyz(xyz, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
-            } else {
-                for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32)
-                    body(c0, neighbor_mask(cur, c0, min(c0 + 32u, b), s, pi, sc.sd.h2));
+            for (uint32_t c0 = rows.a[k]; c0 < b; c0 += 32) {
+                const uint32_t c1 = min(c0 + 32u, b);
+                if constexpr (kXyz) body(c0, neighbor_mask_xyz(xyz, c0, c1, s, pi, sc.sd.h2));
+                else body(c0, neighbor_mask(cur, c0, c1, s, pi, sc.sd.h2));
             }
         }
     } else {
@@ -232,14 +231,15 @@ __device__ __forceinline__ void neighbors_from_chunk(uint32_t s, const float4& p
 // Walks slot s's stencil, keeping its first kLdsChunks chunks (c0, mask)
 // in this lane's LDS column le (stride lst) and chunks kLdsChunks ..
 // kChunkScratch - 1 at their index in nl.spill; returns the number of
-// chunks and the number of neighbours in nbrs.  The candidates come from
-// nl.xyz when the iterate has a packed copy there.
+// chunks and the number of neighbours in nbrs.  With kXyz the candidates
+// come from nl.xyz, the iterate's packed copy.
+template <bool kXyz>
 __device__ __forceinline__ uint32_t record_chunks(uint32_t s, const float4& pi, const SceneDev& sc,
                                                   const float4* __restrict__ cur,
                                                   const uint32_t* __restrict__ cell_start, uint32_t n,
                                                   const NbrList& nl, uint2* le, uint32_t lst, uint32_t& nbrs) {
     uint32_t K = 0, c = 0;
-    for_each_chunk<true>(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
+    for_each_chunk<true, kXyz>(s, pi, sc, cur, cell_start, n, [&](uint32_t c0, uint32_t mask) {
         const uint2 e = make_uint2(c0, mask);
         // two stores under conditions of their own: as if / else if, the two
         // destinations merge into one generic pointer (flat stores)
@@ -515,8 +515,8 @@ struct DeltaSum {
 // D) with D = (|sum_j grad W|^2 + sum_j |grad W|^2) / rho0^2, so that
 // lambda = -C / (D + eps).  With kList the neighbours go to nl; the walk's
 // chunks are staged in this lane's LDS column le (stride lst, kLdsChunks
-// entries).
-template <bool kOrig, bool kCount, bool kTape, bool kList>
+// entries); with kXyz the stencil filter reads nl.xyz.
+template <bool kOrig, bool kCount, bool kTape, bool kList, bool kXyz = false>
 __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& sc, float4* __restrict__ cur,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ sorted_index, uint32_t n,
@@ -535,7 +535,7 @@ __device__ __forceinline__ void density_lambda_slot(uint32_t s, const SceneDev& 
     if (kList) {
         // filter first, keeping the chunks, then the terms over them; the
         // list for the passes that follow is written on the way
-        const uint32_t K = record_chunks(s, pi, sc, cur, cell_start, n, nl, le, lst, cnt);
+        const uint32_t K = record_chunks<kXyz>(s, pi, sc, cur, cell_start, n, nl, le, lst, cnt);
         auto load = [&](uint32_t j) { return cur[j]; };
         auto term = [&](uint32_t, const float4& q) { density_term(pi, q, sd, wsum, gx, gy, gz, gsq); };
         visit_chunks<kLdsChunks>(

@@ -13,7 +13,7 @@ namespace mtlp {
 
 namespace {
 
-template <uint32_t kB, bool kOrig, bool kCount, bool kList, bool kPersist>
+template <uint32_t kB, bool kOrig, bool kCount, bool kList, bool kPersist, bool kXyz>
 __global__ __launch_bounds__(kB) void k_density_lambda(
     float4* __restrict__ cur, const uint32_t* __restrict__ cell_start,
     const uint32_t* __restrict__ sorted_index, StepArgs a, float* __restrict__ lambda_orig,
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kB) void k_density_lambda(
         }
         if (s >= n) return;
         with_scene(s, a, [&](const SceneDev& sc) {
-            density_lambda_slot<kOrig, kCount, false, kList>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
+            density_lambda_slot<kOrig, kCount, false, kList, kXyz>(s, sc, cur, cell_start, sorted_index, n, lambda_orig,
                                                              nbr_orig, nullptr, nl, ent + threadIdx.x, kB);
         });
     };
@@ -147,30 +147,32 @@ __global__ __launch_bounds__(kBlock) void k_mouse_pulse(const float4* __restrict
     }
 }
 
-template <uint32_t kB, bool kList, bool kPersist>
+template <uint32_t kB, bool kList, bool kPersist, bool kXyz>
 void density_lambda_b(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a,
                       float* lo, uint32_t* nb, const NbrList& nl, hipStream_t s) {
     const dim3 grid(stencil_grid(blocks_for(a.n, kB), kB, nl)), block(kB);
     if (lo && nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, true, kList, kPersist>), grid, block, 0, s, cur, cs, sidx, a,
-                           lo, nb, nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, true, kList, kPersist, kXyz>), grid, block, 0, s, cur, cs,
+                           sidx, a, lo, nb, nl);
     else if (lo)
-        hipLaunchKernelGGL((k_density_lambda<kB, true, false, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
-                           a, lo, nb, nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, true, false, kList, kPersist, kXyz>), grid, block, 0, s, cur, cs,
+                           sidx, a, lo, nb, nl);
     else if (nb)
-        hipLaunchKernelGGL((k_density_lambda<kB, false, true, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
-                           a, lo, nb, nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, true, kList, kPersist, kXyz>), grid, block, 0, s, cur, cs,
+                           sidx, a, lo, nb, nl);
     else
-        hipLaunchKernelGGL((k_density_lambda<kB, false, false, kList, kPersist>), grid, block, 0, s, cur, cs, sidx,
-                           a, lo, nb, nl);
+        hipLaunchKernelGGL((k_density_lambda<kB, false, false, kList, kPersist, kXyz>), grid, block, 0, s, cur, cs,
+                           sidx, a, lo, nb, nl);
 }
 
 template <uint32_t kB>
 void density_lambda_bl(float4* cur, const uint32_t* cs, const uint32_t* sidx, const StepArgs& a, float* lo,
                        uint32_t* nb, const NbrList& nl, hipStream_t s) {
-    if (!nl.blk) density_lambda_b<kB, false, false>(cur, cs, sidx, a, lo, nb, nl, s);
-    else if (nl.pull) density_lambda_b<kB, true, true>(cur, cs, sidx, a, lo, nb, nl, s);
-    else density_lambda_b<kB, true, false>(cur, cs, sidx, a, lo, nb, nl, s);
+    if (!nl.blk) density_lambda_b<kB, false, false, false>(cur, cs, sidx, a, lo, nb, nl, s);
+    else if (nl.pull && nl.xyz) density_lambda_b<kB, true, true, true>(cur, cs, sidx, a, lo, nb, nl, s);
+    else if (nl.pull) density_lambda_b<kB, true, true, false>(cur, cs, sidx, a, lo, nb, nl, s);
+    else if (nl.xyz) density_lambda_b<kB, true, false, true>(cur, cs, sidx, a, lo, nb, nl, s);
+    else density_lambda_b<kB, true, false, false>(cur, cs, sidx, a, lo, nb, nl, s);
 }
 
 template <uint32_t kB>

@@ -308,11 +308,17 @@ class Engine:
         tiles at a time from per-XCD counters (same results)."""
         C.check(C.lib().mtlp_set_tile_pull(self._h, int(tiles_per_pull)), "mtlp_set_tile_pull")
 
+    def set_packed_positions(self, on=True):
+        """Density pass filters its candidates from a packed (x, y, z) copy of
+        each iterate (default on; same results)."""
+        C.check(C.lib().mtlp_set_packed_positions(self._h, 1 if on else 0), "mtlp_set_packed_positions")
+
     def launch_config(self):
-        """(stencil workgroup size, tiles per pull)."""
-        b, t = C._U32(), C._U32()
-        C.check(C.lib().mtlp_launch_config(self._h, ctypes.byref(b), ctypes.byref(t)), "mtlp_launch_config")
-        return b.value, t.value
+        """(stencil workgroup size, tiles per pull, packed positions)."""
+        b, t, p = C._U32(), C._U32(), ctypes.c_int()
+        C.check(C.lib().mtlp_launch_config(self._h, ctypes.byref(b), ctypes.byref(t), ctypes.byref(p)),
+                "mtlp_launch_config")
+        return b.value, t.value, bool(p.value)
 
     def set_neighbor_lists(self, on=True):
         """Density pass records neighbour lists for the delta pass (default on)."""

@@ -138,7 +138,7 @@ def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
         blk = e.autotune(DT, cache)
         assert blk in (64, 128, 256, 512)
         cfg = e.launch_config()
-        assert cfg[0] == blk and cfg[1] in (0, 1, 4)
+        assert cfg[0] == blk and cfg[1] in (0, 1, 4) and cfg[2] in (True, False)
         after = _read_all(e)
         after["foam"] = e.foam_intensity_buffer().read()
         for k in before:  # every readable buffer as before the call

@@ -429,11 +429,49 @@ def test_persistent_launches_bit_identical(mtlp, case):
                     e.set_tile_pull(t)
                     e.step(dt)
                     outs[name] += [e.positions_buffer().read(), e.lambdas_buffer().read()]
-                assert e.launch_config() == (block, pulls[-1])
+                assert e.launch_config()[:2] == (block, pulls[-1])
                 assert e.error_flags() == 0
         for name in outs:
             for a, b in zip(outs["dispatch"], outs[name]):
                 np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=(case, block, name))
+
+
+@pytest.mark.parametrize("case", ["blobs_20k", "dam_break_1m"])
+def test_packed_positions_bit_identical(mtlp, case):
+    """The density pass's stencil filter reading the packed (x, y, z) copy of
+    each iterate gives the bits of the filter reading the float4 iterate:
+    positions, lambdas and neighbour counts, with the copy on, off, switched
+    between steps, and in persistent launches; on the blob scene (heavy
+    queues, chunk-scratch walks) and on the 1M dam break."""
+    if case == "blobs_20k":
+        n = 20000
+        p, x, v = _blob_case(mtlp, n)
+        steps = 4
+    else:
+        n = 1_000_000
+        p = mtlp.scene_params(4, n)
+        x, v = mtlp.generate_scene(4, n, 1, p)
+        steps = 2
+    dt = 1.0 / 120.0
+    runs = {"off": [(False, 0)] * steps, "on": [(True, 0)] * steps, "on, pull 4": [(True, 4)] * steps,
+            "switching": [(True, 0), (False, 0), (True, 1), (False, 4)][:steps]}
+    outs = {}
+    for name, cfgs in runs.items():
+        with mtlp.Engine(0, p) as e:
+            e.reset(n, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+            outs[name] = []
+            for packed, pull in cfgs:
+                e.set_packed_positions(packed)
+                e.set_tile_pull(pull)
+                e.step(dt)
+                outs[name] += [e.positions_buffer().read(), e.lambdas_buffer().read(),
+                               e.neighbor_counts_buffer().read()]
+            assert e.launch_config()[1:] == (cfgs[-1][1], cfgs[-1][0])
+            assert e.error_flags() == 0
+    for name in outs:
+        for a, b in zip(outs["off"], outs[name]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32), err_msg=(case, name))
 
 
 def test_step_free_running_vs_fp64(mtlp, orc):
