@@ -148,13 +148,14 @@ hipError_t noalloc_scan_scratch(uint64_t** out);
 //
 // The density pass walks the stencil in chunks of at most 32 consecutive
 // candidate slots, (c0, mask) with bit k set when slot c0 + k is a
-// neighbour, keeping the first kLdsChunks of a slot's chunks in LDS (8
-// bytes each per thread) and chunks kLdsChunks .. kChunkScratch - 1 in
-// spill[k * stride + s], before running the terms over them.  A slot with
-// more than kMaxList neighbours gets count kCountChunks | K and all its K
-// <= kChunkScratch chunks in spill, which the delta pass right after walks
-// (the scratch is rewritten by the next density pass); more chunks than
-// that (none in practice) gives kCountStencil, and readers walk the stencil.
+// neighbour, and keeps only the chunks with a neighbour (walk order): the
+// first kLdsChunks of them in LDS (8 bytes each per thread) and kept chunks
+// kLdsChunks .. kChunkScratch - 1 in spill[k * stride + s], before running
+// the terms over them.  A slot with more than kMaxList neighbours gets count
+// kCountChunks | K and all its K <= kChunkScratch kept chunks in spill,
+// which the delta pass right after walks (the scratch is rewritten by the
+// next density pass); more kept chunks than that (none in practice) gives
+// kCountStencil, and readers walk the stencil.
 constexpr uint32_t kMaxList = 96;
 constexpr uint32_t kListBlocks = kMaxList / 4;
 // 15 rather than 16: with 7,680 bytes of LDS per one-wave workgroup the LDS

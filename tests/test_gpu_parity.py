@@ -257,25 +257,28 @@ def test_step_kernel_level_after_settling(mtlp, orc):
 
 
 def _chunks_and_counts(orc, params, x, v, dt):
-    """Per particle at iteration 0: stencil chunks (rows of <= 32 candidate
-    slots, as the density pass cuts them) and neighbour count."""
+    """Per particle at iteration 0: the stencil chunks the density pass keeps
+    (chunks of <= 32 candidate slots of a stencil row, as it cuts them, that
+    hold a neighbour) and the neighbour count.  The chunks are counted from
+    the float64 neighbour pairs of a k-d tree: a coverage count (a pair at
+    |r| = h within rounding may fall either way), not a parity check."""
+    from scipy.spatial import cKDTree
     op = orc.params(params)
     g = orc.grid(op)
     xs = orc.predict(x, v, op, dt)
     bh = orc.build_spatial_hash(xs, g, np.float32(op.h) * np.float32(op.h))
     cs = bh["cell_start"].astype(np.int64)
-    sx = bh["sorted4"][:, :3]
+    s4 = bh["sorted4"][:, :3]
     dims = np.array(g.dim, np.int64)
-    c = np.clip(np.floor((sx - np.array(op.bounds_min, np.float32)) / np.float32(op.cell_size)).astype(np.int64),
+    c = np.clip(np.floor((s4 - np.array(op.bounds_min, np.float32)) / np.float32(op.cell_size)).astype(np.int64),
                 0, dims - 1)
-    K = np.zeros(len(x), np.int64)
-    for dz in (-1, 0, 1):
-        for dy in (-1, 0, 1):
-            z, y = c[:, 2] + dz, c[:, 1] + dy
-            ok = (z >= 0) & (z < dims[2]) & (y >= 0) & (y < dims[1])
-            x0, x1 = np.maximum(c[:, 0] - 1, 0), np.minimum(c[:, 0] + 1, dims[0] - 1)
-            row = dims[0] * (np.clip(y, 0, dims[1] - 1) + dims[1] * np.clip(z, 0, dims[2] - 1))
-            K += np.where(ok, (cs[row + x1 + 1] - cs[row + x0] + 31) // 32, 0)
+    pairs = cKDTree(s4.astype(np.float64)).query_pairs(float(op.h), output_type="ndarray")
+    i = np.concatenate([pairs[:, 0], pairs[:, 1]])
+    j = np.concatenate([pairs[:, 1], pairs[:, 0]])
+    row = (c[j, 2] - c[i, 2] + 1) * 3 + (c[j, 1] - c[i, 1] + 1)
+    first = cs[np.maximum(c[i, 0] - 1, 0) + dims[0] * (c[j, 1] + dims[1] * c[j, 2])]  # the row's first slot
+    chunk = (i * 9 + row) * 4096 + (j - first) // 32
+    K = np.bincount(np.unique(chunk) // (9 * 4096), minlength=len(x))
     return K, bh["nbr"]
 
 
@@ -326,12 +329,13 @@ def test_dense_regions_chunk_paths(mtlp, orc):
     _kernel_level(mtlp, orc, p, x, v, dt)
 
 
-@pytest.mark.parametrize("n,m", [(40000, 30000), (200000, 100000)])
+@pytest.mark.parametrize("n,m", [(40000, 32000), (200000, 100000)])
 def test_heavy_queue_overflow(mtlp, orc, n, m):
-    """A dense cube of m particles (about 150 neighbours per interior one)
-    gives thousands of slots with over 32 stencil chunks (a wave each from
-    the delta pass's heavy queue) and more slots with over 96 neighbours in
-    at most 32 chunks than its heavy-lane queue holds (n / 8, rounded up to
+    """Two dense cubes, m / 4 particles at 150 per h^3 (about 600
+    neighbours) and 3 m / 4 at 36 per h^3 (about 150), give thousands of
+    slots with over 32 kept stencil chunks (a wave each from the delta
+    pass's heavy queue) and more slots with over 96 neighbours in at most
+    32 kept chunks than its heavy-lane queue holds (n / 8, rounded up to
     4,096): at n = 40,000 that queue (8,192) is short enough for the heavy
     waves to take its slots; at n = 200,000 (28,672) its slots are walked
     one per lane in the heavy-lane workgroups, and both queues overflow into
@@ -342,8 +346,12 @@ def test_heavy_queue_overflow(mtlp, orc, n, m):
     x, v = mtlp.generate_scene(3, n, 5, p)
     rng = np.random.default_rng(11)
     centre = x[:, :3].mean(axis=0)
-    side = (m / 36.0) ** (1.0 / 3.0)  # 36 particles per h^3
-    x[:m, :3] = (centre + rng.uniform(-0.5, 0.5, (m, 3)) * side * p.h).astype(np.float32)
+    m1, m2 = m // 4, m - m // 4
+    s1, s2 = (m1 / 150.0) ** (1.0 / 3.0), (m2 / 36.0) ** (1.0 / 3.0)  # sides in h
+    x[:m1, :3] = (centre + np.array([-0.5 * s2 - 1.0, 0.0, 0.0]) * p.h
+                  + rng.uniform(-0.5, 0.5, (m1, 3)) * s1 * p.h).astype(np.float32)
+    x[m1:m, :3] = (centre + np.array([0.5 * s1 + 1.0, 0.0, 0.0]) * p.h
+                   + rng.uniform(-0.5, 0.5, (m2, 3)) * s2 * p.h).astype(np.float32)
     dt = 1.0 / 120.0
     K, cnt = _chunks_and_counts(orc, p, x, v, dt)
     lanes_cap = (n // 8 + 4095) // 4096 * 4096
