@@ -316,7 +316,8 @@ mtlp_status mtlp_load_state(mtlp_engine* e, const char* path);
 
 /* Steps the engine `steps` times (as mtlp_step) and keeps (x, v) before
  * steps 0, k, 2k, ... (k = ckpt_every) in engine memory for
- * mtlp_adjoint_rollout.  The engine ends at step `steps`. */
+ * mtlp_adjoint_rollout.  The engine ends at step `steps`.  Engines of more
+ * than 2^28 particles get MTLP_E_RANGE. */
 mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps,
                          uint32_t ckpt_every);
 /* Adjoint of the last mtlp_rollout (same dt, steps and ckpt_every): from

@@ -15,6 +15,13 @@ mtlp_status check_single(const mtlp_engine* e) {
     return MTLP_OK;
 }
 
+// The backward stencil kernels index kAdjSub lanes per particle with 32-bit
+// thread numbers.
+mtlp_status check_adjoint_size(const mtlp_engine* e) {
+    if (e->n > kMaxAdjointParticles) return fail(MTLP_E_RANGE, "the adjoint runs on at most 2^28 particles");
+    return MTLP_OK;
+}
+
 mtlp_status check_no_sphere(const mtlp_engine* e) {
     if (e->sphere.on) return fail(MTLP_E_INVALID, "the kinematic sphere's collision is not differentiated");
     return MTLP_OK;
@@ -162,6 +169,7 @@ extern "C" {
 
 mtlp_status mtlp_rollout(mtlp_engine* e, mtlp_stream stream, float dt, uint32_t steps, uint32_t ckpt_every) {
     MTLP_TRY(check_single(e));
+    MTLP_TRY(check_adjoint_size(e));
     MTLP_TRY(check_no_sphere(e));
     MTLP_TRY(check_dt(dt));
     if (steps == 0 || ckpt_every == 0) return fail(MTLP_E_INVALID, "steps and ckpt_every must be >= 1");

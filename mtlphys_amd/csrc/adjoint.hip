@@ -422,6 +422,45 @@ __device__ __forceinline__ float ipow_f(float x, uint32_t n) {
     return result;
 }
 
+// The backward stencil kernels run kAdjSub lanes per slot: a slot's listed
+// neighbours are split between them by list block (lane `sub` takes blocks
+// sub, sub + kAdjSub, ...), and their partial sums are added across the
+// lanes at the end.  With one lane per slot the 64K differentiable workload
+// ran one wave per SIMD, each waiting on its own neighbour loads; four lanes
+// per slot give each SIMD four waves with a quarter of the walk each.
+// A slot whose neighbours are not listed is walked by its first lane alone.
+static_assert(kAdjSub == 4, "group_sum adds over four lanes");
+
+template <class Load, class Pair>
+__device__ __forceinline__ void for_each_neighbor_split(uint32_t s, uint32_t sub, const float4& pi,
+                                                        const SceneDev& sc, const float4* __restrict__ cur,
+                                                        const uint32_t* __restrict__ cell_start, uint32_t n,
+                                                        const NbrList& nl, Load&& load, Pair&& pair) {
+    const uint32_t cnt = nl.count[s];
+    if (cnt <= kMaxList) {
+        const uint4* p = nl.blk + s;
+#pragma unroll 1
+        for (uint32_t b = sub; 4u * b < cnt; b += kAdjSub) {
+            const uint4 e = p[(size_t)b * nl.stride];
+            const auto v0 = load(e.x), v1 = load(e.y), v2 = load(e.z), v3 = load(e.w);
+            pair(e.x, v0);
+            if (4u * b + 1u < cnt) pair(e.y, v1);
+            if (4u * b + 2u < cnt) pair(e.z, v2);
+            if (4u * b + 3u < cnt) pair(e.w, v3);
+        }
+    } else if (sub == 0u) {
+        for_each_neighbor_of<true, false>(s, pi, sc, cur, cell_start, n, nl, load, pair);
+    }
+}
+
+// The sum of v over the kAdjSub lanes of this lane's slot (every lane of the
+// group must be active).
+__device__ __forceinline__ float group_sum(float v) {
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    return v;
+}
+
 // applyDeltaSorted backwards for slot s.  cur = iterate k with lambda_k in
 // .w, bins = iterate 0 (the binned positions), rbar from k_delta_vjp_pre.
 //
@@ -438,8 +477,10 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                                                  const float4* __restrict__ rbar, float* __restrict__ lbar,
                                                  float* __restrict__ lrem, float4* __restrict__ pacc,
                                                  float4* __restrict__ prem, NbrList nl) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-    if (s >= a.n) return;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t s = t / kAdjSub, sub = t % kAdjSub;
+    if (s >= a.n) return;  // the slot's whole lane group
+    float lb = 0.f, px = 0.f, py = 0.f, pz = 0.f;
     with_scene(s, a, [&](const SceneDev& sc) {
         const SolverDev& sd = sc.sd;
         const float4 pi = cur[s], rs = rbar[s], b0 = bins[s];
@@ -449,7 +490,6 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
         const uint32_t sn = sd.scorr_n;
         // d s_corr / d r = dsk * ratio^(n-1) * gp, gp = -6 poly6 (h^2 - r^2)^2
         const float dsk = -sk * (float)sn * inv_wq * (-6.0f * poly6);
-        float lb = 0.f, px = 0.f, py = 0.f, pz = 0.f;
         auto pair = [&](uint32_t c, const float4& pc, const float4& rc) {
             const PairGeo g = pair_geo(pi, pc, sd);
             const float diff = sd.h2 - g.r2;
@@ -477,13 +517,19 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
                 atomic_sub3(prem + c, tx, ty, tz);
             }
         };
-        for_each_neighbor_of<true, false>(
-            s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], rbar[j]}; },
+        for_each_neighbor_split(
+            s, sub, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], rbar[j]}; },
             [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
+    });
+    lb = group_sum(lb);
+    px = group_sum(px);
+    py = group_sum(py);
+    pz = group_sum(pz);
+    if (sub == 0u) {
         lbar[s] = lb;
         const float4 acc = pacc[s];
         pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
-    });
+    }
 }
 
 // densityLambdaSorted backwards for slot s: cur = iterate k with rhobar in
@@ -497,14 +543,15 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
                                                    const uint32_t* __restrict__ cell_start, StepArgs a,
                                                    const float4* __restrict__ db, float4* __restrict__ pacc,
                                                    float4* __restrict__ prem, NbrList nl) {
-    const uint32_t s = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
-    if (s >= a.n) return;
+    const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
+    const uint32_t s = t / kAdjSub, sub = t % kAdjSub;
+    if (s >= a.n) return;  // the slot's whole lane group
+    float px = 0.f, py = 0.f, pz = 0.f;
     with_scene(s, a, [&](const SceneDev& sc) {
         const SolverDev& sd = sc.sd;
         const float4 pi = cur[s], ds4 = db[s], b0 = bins[s];
         const Cell3 bin = cell_of(b0.x, b0.y, b0.z, sc.g);
         const float gp6 = -6.0f * (float)sd.poly6_d;
-        float px = 0.f, py = 0.f, pz = 0.f;
         auto pair = [&](uint32_t c, const float4& pc, const float4& dc) {
             const PairGeo g = pair_geo(pi, pc, sd);
             const float diff = sd.h2 - g.r2;
@@ -524,12 +571,17 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             pz += tz;
             if (!both) atomic_sub3(prem + c, tx, ty, tz);
         };
-        for_each_neighbor_of<true, false>(
-            s, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], db[j]}; },
+        for_each_neighbor_split(
+            s, sub, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], db[j]}; },
             [&](uint32_t j, const Pair4& v) { pair(j, v.a, v.b); });
+    });
+    px = group_sum(px);
+    py = group_sum(py);
+    pz = group_sum(pz);
+    if (sub == 0u) {
         const float4 acc = pacc[s];
         pacc[s] = make_float4(acc.x + px, acc.y + py, acc.z + pz, 0.f);
-    });
+    }
 }
 
 // ---- density at arbitrary positions and its adjoint (fresh binning) ----
@@ -686,12 +738,12 @@ hipError_t launch_step_vjp(const StepTape& t, const AdjointBufs& b, const StepAr
     for (uint32_t k = iterations; k-- > 0;) {
         hipLaunchKernelGGL(k_delta_vjp_pre, pg, pb, 0, s, t.iterate(k), t.delta(k), a, b.pacc, b.prem, b.rbar,
                            b.gdev);
-        MTLP_STENCIL_LAUNCH(k_delta_vjp, block, a.n, s, t.iterate(k), t.iterate(0), t.cell_start, a, b.rbar,
-                            b.lbar, b.lrem, b.pacc, b.prem, t.list(k));
+        MTLP_STENCIL_LAUNCH(k_delta_vjp, block, a.n * kAdjSub, s, t.iterate(k), t.iterate(0), t.cell_start, a,
+                            b.rbar, b.lbar, b.lrem, b.pacc, b.prem, t.list(k));
         hipLaunchKernelGGL(k_density_vjp_pre, pg, pb, 0, s, t.iterate(k), t.grad_sums(k), a, b.lbar, b.lrem,
                            b.db);
-        MTLP_STENCIL_LAUNCH(k_density_vjp, block, a.n, s, t.iterate(k), t.iterate(0), t.cell_start, a, b.db,
-                            b.pacc, b.prem, t.list(k));
+        MTLP_STENCIL_LAUNCH(k_density_vjp, block, a.n * kAdjSub, s, t.iterate(k), t.iterate(0), t.cell_start, a,
+                            b.db, b.pacc, b.prem, t.list(k));
     }
     hipLaunchKernelGGL(k_predict_vjp, pg, pb, 0, s, b.pacc, b.prem, t.sorted_index, a, xbar_in, vbar_in,
                        b.gdev, b.gpart, b.err);

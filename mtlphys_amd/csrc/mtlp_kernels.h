@@ -279,6 +279,10 @@ hipError_t launch_mouse_pulse(const float4* x, float4* v, uint32_t n, const floa
                               const float dir[3], const float force[3], float radius, hipStream_t s);
 
 // ---- adjoint (adjoint.hip) ---------------------------------------------
+// Lanes per particle of the backward stencil kernels, and the particle
+// count their 32-bit thread numbers allow.
+constexpr uint32_t kAdjSub = 4;
+constexpr uint32_t kMaxAdjointParticles = 1u << 28;
 // What the adjoint of one step reads from its taped forward: the hash, the
 // iterates p^0 .. p^I (sorted float4; p^k.w = lambda^k for k < I), per
 // iteration the density pass's gradient sums (sum_j grad W, D) and the
