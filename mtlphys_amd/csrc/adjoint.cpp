@@ -75,6 +75,8 @@ mtlp_status ensure_work(mtlp_engine* e) {
     HIP_TRY(dev_alloc(&A.db, st));
     HIP_TRY(dev_alloc(&A.lbar, st));
     HIP_TRY(dev_alloc(&A.lrem, st));
+    HIP_TRY(dev_alloc(&A.rcnt, st));
+    HIP_TRY(dev_alloc(&A.rval, kInbox * st));
     HIP_TRY(dev_alloc(&A.gdev, 6 * st));
     HIP_TRY(dev_alloc(&A.gpart, 6 * blocks));
     HIP_TRY(dev_alloc(&A.gstate, 9));
@@ -88,6 +90,7 @@ mtlp_status ensure_work(mtlp_engine* e) {
     // the remainder buffers start at zero; every backward step leaves them so
     HIP_TRY(hipMemset(A.prem, 0, sizeof(float4) * st));
     HIP_TRY(hipMemset(A.lrem, 0, sizeof(float) * st));
+    HIP_TRY(hipMemset(A.rcnt, 0, sizeof(uint32_t) * st));
     A.work_stride = st;
     return MTLP_OK;
 }
@@ -249,7 +252,8 @@ mtlp_status mtlp_adjoint_rollout(mtlp_engine* e, mtlp_stream stream, float dt, u
     else HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(A.xb[0]), 0u, 4 * (size_t)e->n, s));
     if (d_vbar_T) HIP_TRY(copy_d2d(A.vb[0], d_vbar_T, bytes, s));
     else HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(A.vb[0]), 0u, 4 * (size_t)e->n, s));
-    AdjointBufs b{A.pacc, A.prem, A.rbar, A.db, A.lbar, A.lrem, A.gdev, A.gpart, A.gstate, e->misc + 2};
+    AdjointBufs b{A.pacc, A.prem, A.rbar, A.db, A.lbar, A.lrem, A.gdev, A.gpart, A.gstate, e->misc + 2,
+                  A.rcnt, A.rval};
     HIP_TRY(launch_gravity_init(b, A.xb[0], A.vb[0], e->n, A.gout, s));
     const StepArgs a = e->step_args(dt, true);
     const uint32_t n_seg = (steps + ckpt_every - 1) / ckpt_every;

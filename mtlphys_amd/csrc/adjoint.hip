@@ -12,14 +12,15 @@
 // kernel for slot s walks its own candidates c and handles both roles:
 //   A (i = s, j = c): s keeps its part; c's part is delivered by c itself
 //     when s is one of c's candidates (bin(s) in the stencil of c's cell),
-//     otherwise by a float atomic into a remainder buffer (rare);
+//     otherwise as a remainder term into c's inbox (rare);
 //   B (i = c, j = s): s takes c's term on s when s is in N_c, i.e. when
 //     bin(s) lies in the stencil of c's current cell.
 // The two roles share the pair's geometry (H is even in r, G odd), so each
 // kernel evaluates them as one expression.  Terms and sums are FP32 (the
 // terms' own rounding dominates; FP64 sums measured no closer to the FP64
-// adjoint); a slot's own result is a plain store, so the output is
-// deterministic up to the order of the rare remainder atomics.
+// adjoint); a slot's own result is a plain store and its inbox is added up
+// independently of arrival order, so the output is deterministic (unless a
+// slot receives more than kInbox remainder terms in one pass).
 //
 // Math (r = p_i - p_j, f(r) = -k_s (h - r)^2 the spiky gradient magnitude,
 // G = f r^ (Constraints.metal:35-39), its Hessian H v = (f' - f/r)(r^.v) r^
@@ -198,6 +199,73 @@ __global__ __launch_bounds__(kBlock) void k_finalize_vjp(
     });
 }
 
+// A remainder term v = (lambda-bar, pbar.xyz) for slot c: into c's inbox,
+// or past its kInbox entries by float atomics (the lambda part only when
+// with_l).
+__device__ __forceinline__ void remainder_put(uint32_t* __restrict__ rcnt, float4* __restrict__ rval,
+                                              float* __restrict__ lrem, float4* __restrict__ prem, uint32_t c,
+                                              const float4& v, bool with_l) {
+    const uint32_t q = atomicAdd(rcnt + c, 1u);
+    if (q < kInbox) {
+        rval[(size_t)c * kInbox + q] = v;
+        return;
+    }
+    if (with_l) atomicAdd(lrem + c, v.x);
+    float* p = reinterpret_cast<float*>(prem + c);
+    atomicAdd(p + 0, v.y);
+    atomicAdd(p + 1, v.z);
+    atomicAdd(p + 2, v.w);
+}
+
+// Order-independent sums of components kFirst .. kFirst + kC - 1 of the
+// m <= kInbox entries e[0..m): every FP32 value of a component is scaled by 2^S to a 64-bit integer, S set by the largest
+// exponent in the component (|total| < 2^61), the integers are added and the
+// total is rounded back to FP32 once.  Any arrival order gives the same bits;
+// parts below 2^-57 of the largest value are truncated toward zero.  A
+// component holding an infinity or NaN is summed as floats (the result is
+// not finite either way).
+__device__ __forceinline__ uint32_t exp_bits(float v) { return (__float_as_uint(v) >> 23) & 0xffu; }
+
+__device__ __forceinline__ double pow2_d(int e) {  // 2^e, -1022 <= e <= 1023
+    return __longlong_as_double((long long)(e + 1023) << 52);
+}
+
+template <int kC, int kFirst>
+__device__ __forceinline__ void inbox_sum(const float4* __restrict__ e, uint32_t m, float* out) {
+    uint32_t emax[kC];
+#pragma unroll
+    for (int k = 0; k < kC; ++k) emax[k] = 1u;
+#pragma unroll 1
+    for (uint32_t q = 0; q < m; ++q) {
+        const float4 v = e[q];
+        const float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < kC; ++k) emax[k] = max(emax[k], exp_bits(c[kFirst + k]));
+    }
+    long long acc[kC];
+    float fsum[kC];
+    double scale[kC];
+#pragma unroll
+    for (int k = 0; k < kC; ++k) {
+        acc[k] = 0;
+        fsum[k] = 0.f;
+        scale[k] = pow2_d(183 - (int)min(emax[k], 254u));
+    }
+#pragma unroll 1
+    for (uint32_t q = 0; q < m; ++q) {
+        const float4 v = e[q];
+        const float c[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < kC; ++k) {
+            const float x = c[kFirst + k];
+            if (emax[k] < 255u) acc[k] += (long long)((double)x * scale[k]);
+            fsum[k] += x;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kC; ++k) out[k] = emax[k] < 255u ? (float)((double)acc[k] / scale[k]) : fsum[k];
+}
+
 // Box clamp and 0.3 h cap of applyDeltaSorted (Constraints.metal:185-197)
 // backwards: pbar of iterate k+1 (own part in pacc plus the remainder) ->
 // Rbar_s, and the pass-through part of pbar of iterate k (into pacc).  A
@@ -205,13 +273,18 @@ __global__ __launch_bounds__(kBlock) void k_finalize_vjp(
 __global__ __launch_bounds__(kBlock) void k_delta_vjp_pre(const float4* __restrict__ cur,
                                                          const float4* __restrict__ rr, StepArgs a,
                                                          float4* __restrict__ pacc, float4* __restrict__ prem,
+                                                         uint32_t* __restrict__ rcnt, const float4* __restrict__ rval,
                                                          float4* __restrict__ rbar, double* __restrict__ gdev) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         const SolverDev& sd = sc.sd;
-        const float4 pb4 = pacc[s], rm = prem[s], p4 = cur[s], r4 = rr[s];
+        const float4 pb4 = pacc[s], ro = prem[s], p4 = cur[s], r4 = rr[s];
+        float in[3];
+        inbox_sum<3, 1>(rval + (size_t)s * kInbox, min(rcnt[s], kInbox), in);
+        const float4 rm = make_float4(in[0] + ro.x, in[1] + ro.y, in[2] + ro.z, 0.f);
         prem[s] = zero4();
+        rcnt[s] = 0u;
         const double R[3] = {r4.x, r4.y, r4.z};
         const double rl2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
         const double rl = sqrt(rl2);
@@ -245,12 +318,15 @@ __global__ __launch_bounds__(kBlock) void k_delta_vjp_pre(const float4* __restri
 __global__ __launch_bounds__(kBlock) void k_density_vjp_pre(float4* __restrict__ cur,
                                                            const float4* __restrict__ aux, StepArgs a,
                                                            const float* __restrict__ lbar,
-                                                           float* __restrict__ lrem, float4* __restrict__ db) {
+                                                           float* __restrict__ lrem, const uint32_t* __restrict__ rcnt,
+                                                           const float4* __restrict__ rval, float4* __restrict__ db) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= a.n) return;
     with_scene(s, a, [&](const SceneDev& sc) {
         const SolverDev& sd = sc.sd;
-        const float lb = lbar[s] + lrem[s];
+        float in;
+        inbox_sum<1, 0>(rval + (size_t)s * kInbox, min(rcnt[s], kInbox), &in);
+        const float lb = lbar[s] + (in + lrem[s]);
         lrem[s] = 0.f;
         const float4 ax = aux[s];
         const float lam = cur[s].w;
@@ -268,7 +344,8 @@ __global__ __launch_bounds__(kBlock) void k_density_vjp_pre(float4* __restrict__
 // this block's sums of gdev (6 doubles) for the gravity recursion.  A
 // non-finite xbar_in (the adjoint outgrew FP32) raises kErrAdjointRange.
 __global__ __launch_bounds__(kBlock) void k_predict_vjp(const float4* __restrict__ pacc,
-                                                       float4* __restrict__ prem,
+                                                       float4* __restrict__ prem, uint32_t* __restrict__ rcnt,
+                                                       const float4* __restrict__ rval,
                                                        const uint32_t* __restrict__ sorted_index, StepArgs a,
                                                        float4* __restrict__ xbar_in, float4* __restrict__ vbar_in,
                                                        const double* __restrict__ gdev, double* __restrict__ gpart,
@@ -276,9 +353,12 @@ __global__ __launch_bounds__(kBlock) void k_predict_vjp(const float4* __restrict
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (s < a.n) {
-        const float4 pb = pacc[s], rm = prem[s];
+        const float4 pb = pacc[s], ro = prem[s];
+        float in[3];
+        inbox_sum<3, 1>(rval + (size_t)s * kInbox, min(rcnt[s], kInbox), in);
         prem[s] = zero4();
-        const float b[3] = {pb.x + rm.x, pb.y + rm.y, pb.z + rm.z};
+        rcnt[s] = 0u;
+        const float b[3] = {pb.x + (in[0] + ro.x), pb.y + (in[1] + ro.y), pb.z + (in[2] + ro.z)};
         const uint32_t i = sorted_index[s];
         const float4 xp = xbar_in[i];
         const float4 xn = make_float4(xp.x + b[0], xp.y + b[1], xp.z + b[2], 0.f);
@@ -405,13 +485,6 @@ __device__ __forceinline__ bool in_stencil(const Cell3& bin, const float4& p, co
     return abs(bin.x - c.x) <= 1 && abs(bin.y - c.y) <= 1 && abs(bin.z - c.z) <= 1;
 }
 
-__device__ __forceinline__ void atomic_sub3(float4* p, float x, float y, float z) {
-    float* q = reinterpret_cast<float*>(p);
-    atomicAdd(q + 0, -x);
-    atomicAdd(q + 1, -y);
-    atomicAdd(q + 2, -z);
-}
-
 __device__ __forceinline__ float ipow_f(float x, uint32_t n) {
     float result = 1.0f, b = x;
     while (n) {
@@ -468,15 +541,16 @@ __device__ __forceinline__ float group_sum(float v) {
 // or Rbar_s (role A only), H even and G odd in r, the pair adds
 //   lbar_s += f (r^.dR) / rho0,
 //   pbar_s += c H dR / rho0 + (f (r^.dR) / rho0) grad s,
-// and without role B, c's share of role A (the same values) goes out as
-// remainder atomics.
+// and without role B, c's share of role A (the same values, negated) goes
+// to c's inbox.
 template <uint32_t kB>
 __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur,
                                                  const float4* __restrict__ bins,
                                                  const uint32_t* __restrict__ cell_start, StepArgs a,
                                                  const float4* __restrict__ rbar, float* __restrict__ lbar,
                                                  float* __restrict__ lrem, float4* __restrict__ pacc,
-                                                 float4* __restrict__ prem, NbrList nl) {
+                                                 float4* __restrict__ prem, uint32_t* __restrict__ rcnt,
+                                                 float4* __restrict__ rval, NbrList nl) {
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t s = t / kAdjSub, sub = t % kAdjSub;
     if (s >= a.n) return;  // the slot's whole lane group
@@ -512,10 +586,7 @@ __global__ __launch_bounds__(kB) void k_delta_vjp(const float4* __restrict__ cur
             px += tx;
             py += ty;
             pz += tz;
-            if (!both) {
-                atomicAdd(lrem + c, gr);
-                atomic_sub3(prem + c, tx, ty, tz);
-            }
+            if (!both) remainder_put(rcnt, rval, lrem, prem, c, make_float4(gr, -tx, -ty, -tz), true);
         };
         for_each_neighbor_split(
             s, sub, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], rbar[j]}; },
@@ -542,7 +613,8 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
                                                    const float4* __restrict__ bins,
                                                    const uint32_t* __restrict__ cell_start, StepArgs a,
                                                    const float4* __restrict__ db, float4* __restrict__ pacc,
-                                                   float4* __restrict__ prem, NbrList nl) {
+                                                   float4* __restrict__ prem, uint32_t* __restrict__ rcnt,
+                                                   float4* __restrict__ rval, NbrList nl) {
     const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * kB + threadIdx.x;
     const uint32_t s = t / kAdjSub, sub = t % kAdjSub;
     if (s >= a.n) return;  // the slot's whole lane group
@@ -569,7 +641,7 @@ __global__ __launch_bounds__(kB) void k_density_vjp(const float4* __restrict__ c
             px += tx;
             py += ty;
             pz += tz;
-            if (!both) atomic_sub3(prem + c, tx, ty, tz);
+            if (!both) remainder_put(rcnt, rval, nullptr, prem, c, make_float4(0.f, -tx, -ty, -tz), false);
         };
         for_each_neighbor_split(
             s, sub, pi, sc, cur, cell_start, a.n, nl, [&](uint32_t j) { return Pair4{cur[j], db[j]}; },
@@ -736,17 +808,17 @@ hipError_t launch_step_vjp(const StepTape& t, const AdjointBufs& b, const StepAr
     hipLaunchKernelGGL(k_finalize_vjp, pg, pb, 0, s, t.iterate(iterations), t.sorted_index, a, t.x_in,
                        xbar_out, vbar_out, b.pacc, xbar_in, b.gdev);
     for (uint32_t k = iterations; k-- > 0;) {
-        hipLaunchKernelGGL(k_delta_vjp_pre, pg, pb, 0, s, t.iterate(k), t.delta(k), a, b.pacc, b.prem, b.rbar,
-                           b.gdev);
+        hipLaunchKernelGGL(k_delta_vjp_pre, pg, pb, 0, s, t.iterate(k), t.delta(k), a, b.pacc, b.prem, b.rcnt,
+                           b.rval, b.rbar, b.gdev);
         MTLP_STENCIL_LAUNCH(k_delta_vjp, block, a.n * kAdjSub, s, t.iterate(k), t.iterate(0), t.cell_start, a,
-                            b.rbar, b.lbar, b.lrem, b.pacc, b.prem, t.list(k));
+                            b.rbar, b.lbar, b.lrem, b.pacc, b.prem, b.rcnt, b.rval, t.list(k));
         hipLaunchKernelGGL(k_density_vjp_pre, pg, pb, 0, s, t.iterate(k), t.grad_sums(k), a, b.lbar, b.lrem,
-                           b.db);
+                           b.rcnt, b.rval, b.db);
         MTLP_STENCIL_LAUNCH(k_density_vjp, block, a.n * kAdjSub, s, t.iterate(k), t.iterate(0), t.cell_start, a,
-                            b.db, b.pacc, b.prem, t.list(k));
+                            b.db, b.pacc, b.prem, b.rcnt, b.rval, t.list(k));
     }
-    hipLaunchKernelGGL(k_predict_vjp, pg, pb, 0, s, b.pacc, b.prem, t.sorted_index, a, xbar_in, vbar_in,
-                       b.gdev, b.gpart, b.err);
+    hipLaunchKernelGGL(k_predict_vjp, pg, pb, 0, s, b.pacc, b.prem, b.rcnt, b.rval, t.sorted_index, a, xbar_in,
+                       vbar_in, b.gdev, b.gpart, b.err);
     const double dt = (double)a.dt;
     hipLaunchKernelGGL(k_reduce_gravity, dim3(1), pb, 0, s, b.gpart, pg.x, dt, (1.0 / dt) * (double)damping,
                        b.gstate, gout);

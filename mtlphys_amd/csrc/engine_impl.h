@@ -99,6 +99,8 @@ struct AdjointState {
     size_t work_stride = 0;
     float4 *pacc = nullptr, *prem = nullptr, *rbar = nullptr, *db = nullptr;
     float *lbar = nullptr, *lrem = nullptr;
+    uint32_t* rcnt = nullptr;  // remainder inbox counters (stride)
+    float4* rval = nullptr;    // remainder inbox entries (kInbox * stride)
     double *gdev = nullptr, *gpart = nullptr, *gstate = nullptr;
     float* gout = nullptr;
     float4* xb[2] = {nullptr, nullptr};
@@ -117,6 +119,7 @@ struct AdjointState {
     }
     void release_work() {
         dev_free(pacc); dev_free(prem); dev_free(rbar); dev_free(db); dev_free(lbar); dev_free(lrem);
+        dev_free(rcnt); dev_free(rval);
         dev_free(gdev); dev_free(gpart); dev_free(gstate); dev_free(gout);
         dev_free(xb[0]); dev_free(xb[1]); dev_free(vb[0]); dev_free(vb[1]);
         work_stride = 0;

@@ -312,20 +312,34 @@ struct StepTape {
     }
 };
 
-// Work buffers of the backward pass (sorted order unless noted).  prem and
-// lrem receive the remainder atomics and must be zero between steps (the
-// pointwise stages that consume them clear them).
+// Remainder terms (a pair's share for a slot that cannot see the pair) go
+// to the receiving slot's inbox: kInbox entries (lambda-bar, pbar.xyz) per
+// slot, appended at positions handed out by an atomic counter and added up
+// by the consuming stage in an order-independent way, so the adjoint's bits
+// do not depend on the order of the atomics.  Entries past kInbox go to
+// float atomics in prem / lrem instead (then the order matters again).
+// Each such pair sends two terms per iteration (delta and density passes):
+// the 64K differentiable workload needs at most 12 entries per pass, the
+// 1M dam break with the benchmark constants up to 32 in its first 15 steps
+// (FP32 oracle trajectories).
+constexpr uint32_t kInbox = 32;
+
+// Work buffers of the backward pass (sorted order unless noted).  The
+// inbox counters, prem and lrem must be zero between steps (the pointwise
+// stages that consume them clear them).
 struct AdjointBufs {
     float4* pacc;   // adjoint of the current iterate, own part
-    float4* prem;   // its remainder part
+    float4* prem;   // its remainder part past the inbox
     float4* rbar;   // adjoint of R
     float4* db;     // (2 qbar g, qbar)
     float* lbar;    // adjoint of lambda, own part
-    float* lrem;    // its remainder part
+    float* lrem;    // its remainder part past the inbox
     double* gdev;   // 6 per slot: deviation terms of the gravity recursion
     double* gpart;  // 6 per block of kBlock slots
     double* gstate; // 9: sum xbar, sum vbar, the gravity gradient so far
     uint32_t* err;  // error flag word
+    uint32_t* rcnt; // per slot: inbox entries appended (may exceed kInbox)
+    float4* rval;   // kInbox entries per slot (slot-major)
 };
 
 // lambda_orig / nbr_orig as in launch_density_lambda (may be NULL).
