@@ -26,8 +26,10 @@ One JSON line on stdout:
                 profiles/valu_dam_break_1m.txt) over its mean duration, against
                 1,024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
   hash_scan     hash + count + scan + stable scatter + gather on 4,194,304
-                points in a 128^3 grid (uniform and clustered), the scan alone
-                at 2^20 .. 2^24 elements, and buildSpatialHash on the dam break
+                points in a 128^3 grid (uniform, clustered, and 95% of them in
+                200 cells of ~19,900: the ranking of huge cells), the scan
+                alone at 2^20 .. 2^24 elements, and buildSpatialHash on the
+                dam break
   parity        per-output errors against the FP64 oracle at this workload,
                 as recorded by tests/test_gpu_parity_1m.py
                 (profiles/parity_dam_break_1m.json)
@@ -313,6 +315,21 @@ def _hash_stages(eng, reps, stream, warm=3):
     return {k: statistics.fmean(v) for k, v in per.items()}
 
 
+def _hash_step_stages(eng, reps, stream, warm=3):
+    """Mean us per stage of `reps` steps with no solver iteration (the hash
+    of the current positions without countNeighbors) (HIP events)."""
+    for _ in range(warm):
+        eng.step(DT, stream)
+    eng.set_profiling(True)
+    per = {}
+    for _ in range(reps):
+        eng.step(DT, stream)
+        for name, ms in eng.stage_times():
+            per.setdefault(name, []).append(ms * 1e3)
+    eng.set_profiling(False)
+    return {k: statistics.fmean(v) for k, v in per.items()}
+
+
 def hash_scan_leg(torch, dam_state=None, reps=20):
     """The isolated spatial hash (4,194,304 points, 128^3 cells, uniform and
     clustered), the scan alone at 2^20 .. 2^24 elements, and buildSpatialHash
@@ -323,12 +340,18 @@ def hash_scan_leg(torch, dam_state=None, reps=20):
     out = {"hash_128cubed": {}, "scan_only": [], "dam_break_1m_build_spatial_hash": None}
     p = W.unit_grid_params()
     hash_stages = ("predict_hash_count", "exclusive_scan", "stable_scatter_gather")
-    for variant in ("uniform", "clustered"):
-        pts = W.uniform_points(W.HASH_POINTS, 2) if variant == "uniform" else W.clustered_points(W.HASH_POINTS, 2)
-        eng = mtlp.Engine(0, p)
+    gen = {"uniform": W.uniform_points, "clustered": W.clustered_points, "huge_cells": W.huge_cell_points}
+    for variant in ("uniform", "clustered", "huge_cells"):
+        pts = gen[variant](W.HASH_POINTS, 2)
+        q = p.copy()
+        if variant == "huge_cells":  # no countNeighbors: its walk is quadratic in a cell's size
+            q.iterations = 0
+            for ax in range(3):
+                q.gravity[ax] = 0.0
+        eng = mtlp.Engine(0, q)
         eng.reset(W.HASH_POINTS, mtlp.capi.SCENE_EMPTY)
         eng.upload_state(pts, np.zeros_like(pts))
-        st = _hash_stages(eng, reps, stream)
+        st = _hash_stages(eng, reps, stream) if variant != "huge_cells" else _hash_step_stages(eng, reps, stream)
         flags = eng.error_flags()
         cells = eng.total_cells
         eng.close()

@@ -219,6 +219,9 @@ mtlp_status alloc_particles(mtlp_engine* e, uint32_t n) {
         HIP_TRY(hipMemset(e->xyz[1], 0, 3 * sizeof(float) * (na + kChunkSlack)));
     }
     HIP_TRY(dev_alloc(&e->big_list, na / kSmallCell + 1));
+    HIP_TRY(dev_alloc(&e->hcnt, na));
+    HIP_TRY(dev_alloc(&e->hoff, na));
+    HIP_TRY(hipMemset(e->hcnt, 0, sizeof(uint32_t) * na));
     HIP_TRY(dev_alloc(&e->nlist, (size_t)kListBlocks * na));
     HIP_TRY(dev_alloc(&e->ncount, na));
     HIP_TRY(dev_alloc(&e->nspill, (size_t)kChunkScratch * na));

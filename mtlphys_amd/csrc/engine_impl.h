@@ -168,6 +168,8 @@ struct mtlp_engine {
     float* xyz[2] = {nullptr, nullptr};  // packed (x, y, z) of sorted[0], sorted[1] (NbrList::xyz)
     bool packed_positions = true;        // filter from xyz (mtlp_set_packed_positions); off without copies
     uint32_t* big_list = nullptr;
+    uint32_t* hcnt = nullptr;  // huge cells: member counts per index block (zero between builds)
+    uint32_t* hoff = nullptr;  // huge cells: starting ranks per index block
     int final_buf = 0;
     // neighbour list of the current iterate (kListBlocks * n_alloc) and counts
     uint4* nlist = nullptr;
@@ -194,7 +196,8 @@ struct mtlp_engine {
     uint32_t* cell_start = nullptr;
     uint64_t* scan_state = nullptr;
     uint32_t scan_epoch = 0;
-    // misc device words: [0] big-cell count, [1] scan ticket, [2] error flags
+    // misc device words: [0] big-cell count, [1] scan ticket, [2] error
+    // flags, [3] huge-cell flag
     uint32_t* misc = nullptr;
 
     // profiling
@@ -222,7 +225,7 @@ struct mtlp_engine {
     void free_particles() {
         dev_free(x); dev_free(v); dev_free(prev); dev_free(keys); dev_free(tickets);
         dev_free(nbr); dev_free(lambdas); dev_free(foam); dev_free(tmp_index); dev_free(sorted_index);
-        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(xyz[0]); dev_free(xyz[1]); dev_free(slot_of); dev_free(big_list); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]); dev_free(tile_ctr);
+        dev_free(sorted[0]); dev_free(sorted[1]); dev_free(xyz[0]); dev_free(xyz[1]); dev_free(slot_of); dev_free(big_list); dev_free(hcnt); dev_free(hoff); dev_free(nlist); dev_free(ncount); dev_free(nspill); dev_free(heavy); dev_free(heavy_count); dev_free(lanes); dev_free(tile_plan); dev_free(tile_order[0]); dev_free(tile_order[1]); dev_free(tile_ctr);
         tile_order_tiles[0] = tile_order_tiles[1] = 0;
         n_alloc = 0;
     }
@@ -251,6 +254,7 @@ struct mtlp_engine {
         b.sorted = sorted[0]; b.big_list = big_list; b.big_count = misc + 0; b.err = misc + 2;
         b.slot_of = slot_of;
         b.heavy_count = heavy_count;
+        b.hcnt = hcnt; b.hoff = hoff; b.huge_flag = misc + 3;
         b.xyz = use_lists && packed_positions ? xyz[0] : nullptr;  // iteration 0's copy (may be NULL)
         return b;
     }

@@ -16,7 +16,7 @@
 //     entry to cell_start + rank, so the random accesses of the pair are
 //     the scatter's stores rather than gathers.  Cells larger than
 //     kSmallCell are listed and sorted by a separate kernel (LDS bitonic
-//     sort, or an ordered key scan past kLdsSortMax).
+//     sort; past kLdsSortMax a rank per index block, see kHugeBlocks).
 #include "mtlp_device.h"
 
 namespace mtlp {
@@ -26,9 +26,12 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
     const float4* __restrict__ x, const float4* __restrict__ v, StepArgs a,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ tickets, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count) {
+    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ huge_flag) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) *big_count = 0u;  // the big-cell list of this build starts empty
+    if (i == 0) {  // the big-cell list of this build starts empty, with no huge cell
+        *big_count = 0u;
+        *huge_flag = 0u;
+    }
     if (i < 4 && heavy_count) heavy_count[i] = 0u;  // so do the heavy-slot queues of its iterations
     if (i >= a.n) return;
     const float4 xi = x[i];
@@ -51,39 +54,90 @@ __global__ __launch_bounds__(kBlock) void k_scatter_ticket(
     const uint32_t* __restrict__ keys, const uint32_t* __restrict__ tickets,
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, uint2* __restrict__ tmp, float4* __restrict__ stage,
-    uint32_t* __restrict__ err) {
+    uint32_t* __restrict__ err, uint32_t* __restrict__ hcnt) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = a.n;
     if (i >= n) return;
     const uint32_t key = keys[i];
-    const uint32_t d = cell_start[key] + tickets[i];
+    const CellRange r = cell_range(cell_start, key);
+    const uint32_t lo = r.lo;
+    const uint32_t d = lo + tickets[i];
     if (d < n) {
         tmp[d] = make_uint2(i, key);
         stage[d] = binned_pos(x, v, i, a);
+        // a huge cell's members, counted per index block in the cell's
+        // first kHugeBlocks slots (i / block length < kHugeBlocks <= len)
+        const uint32_t len = r.hi - lo;
+        if (len > kLdsSortMax && len <= n - lo) atomicAdd(hcnt + lo + i / huge_block_len(n), 1u);
     } else {
         atomicOr(err, kErrHashRange);
     }
+}
+
+// The wave holding the first slot L of a huge cell turns the cell's member
+// counts per index block (hcnt[L .. L + kHugeBlocks)) into exclusive
+// prefix sums (hoff, the rank of each block's first member), clears the
+// counts for the next build and raises huge_flag.  Every lane takes part.
+__device__ __forceinline__ void huge_offsets(uint32_t* __restrict__ hcnt, uint32_t* __restrict__ hoff,
+                                             uint32_t L, uint32_t* __restrict__ huge_flag) {
+    constexpr uint32_t kPer = kHugeBlocks / 64u;
+    const uint32_t lane = __lane_id();
+    uint32_t* c = hcnt + L + kPer * lane;
+    uint32_t* o = hoff + L + kPer * lane;
+    uint32_t sum = 0;
+#pragma unroll 1
+    for (uint32_t k = 0; k < kPer; ++k) sum += c[k];
+    uint32_t pre = sum;  // inclusive prefix over the lanes
+#pragma unroll
+    for (int w = 1; w < 64; w <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)pre, w);
+        if ((int)lane >= w) pre += u;
+    }
+    pre -= sum;
+#pragma unroll 1
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t ck = c[k];
+        o[k] = pre;
+        c[k] = 0u;
+        pre += ck;
+    }
+    if (lane == 0) *huge_flag = 1u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rank_gather(
     const uint2* __restrict__ tmp, const float4* __restrict__ stage,
     const uint32_t* __restrict__ cell_start, uint32_t n, uint32_t* __restrict__ sorted_index,
     float4* __restrict__ sorted, uint32_t* __restrict__ big_list, uint32_t* __restrict__ big_count,
-    uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of, float* __restrict__ xyz) {
+    uint32_t* __restrict__ err, uint32_t* __restrict__ slot_of, float* __restrict__ xyz,
+    uint32_t* __restrict__ hcnt, uint32_t* __restrict__ hoff, uint32_t* __restrict__ huge_flag) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
-    const uint2 e = tmp[s];
-    const uint32_t i = e.x, key = e.y;
-    if (i >= n) {
-        atomicOr(err, kErrHashRange);
-        return;
+    uint32_t i = 0, key = 0, lo = 0, hi = 0;
+    bool ok = s < n;
+    if (ok) {
+        const uint2 e = tmp[s];
+        i = e.x;
+        key = e.y;
+        if (i >= n) {
+            atomicOr(err, kErrHashRange);
+            ok = false;
+        }
     }
-    const uint32_t lo = cell_start[key], hi = cell_start[key + 1];
-    if (hi > n || lo > hi) {
-        atomicOr(err, kErrHashRange);
-        return;
+    if (ok) {
+        const CellRange r = cell_range(cell_start, key);
+        lo = r.lo;
+        hi = r.hi;
+        if (hi > n || lo > hi) {
+            atomicOr(err, kErrHashRange);
+            ok = false;
+        }
     }
-    if (hi - lo > kSmallCell) {
+    const bool big = ok && hi - lo > kSmallCell;
+    // a huge cell starting in this wave (at most one: it spans > kLdsSortMax
+    // slots): the whole wave turns its per-block counts into ranks
+    const uint64_t hs = __ballot(big && s == lo && hi - lo > kLdsSortMax);
+    if (hs) huge_offsets(hcnt, hoff, (uint32_t)__builtin_amdgcn_readlane((int)lo, (int)__builtin_ctzll(hs)), huge_flag);
+    if (!ok) return;
+    if (big) {
         if (s == lo) big_list[atomicAdd(big_count, 1u)] = key;
         return;
     }
@@ -103,9 +157,9 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     const uint32_t* __restrict__ cell_start, const float4* __restrict__ x,
     const float4* __restrict__ v, StepArgs a, const uint32_t* __restrict__ big_list,
     const uint32_t* __restrict__ big_count, uint32_t* __restrict__ sorted_index,
-    float4* __restrict__ sorted, uint32_t* __restrict__ slot_of, float* __restrict__ xyz) {
+    float4* __restrict__ sorted, uint32_t* __restrict__ slot_of, float* __restrict__ xyz,
+    uint32_t* __restrict__ hoff, const uint32_t* __restrict__ huge_flag) {
     __shared__ uint32_t lds[kLdsSortMax];
-    __shared__ uint32_t s_wave[kBigBlock / 64];
     const uint32_t t = threadIdx.x;
     const uint32_t lane = t & 63u, wave = t >> 6;
     const uint32_t n = a.n;
@@ -148,34 +202,57 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
                 if (slot_of) slot_of[i] = lo + k;
             }
             __syncthreads();
-        } else {
-            // Rank by an ordered pass over all keys: the r-th particle (by
-            // index) whose key matches goes to slot lo + r.
-            uint32_t run = 0;
-            for (uint32_t base = 0; base < n; base += kBigBlock) {
-                const uint32_t j = base + t;
-                const bool hit = (j < n) && keys[j] == key;
-                const uint64_t m = __ballot(hit);
-                if (lane == 0) s_wave[wave] = (uint32_t)__popcll(m);
-                __syncthreads();
-                uint32_t pre = 0, tot = 0;
-                for (uint32_t w = 0; w < kBigBlock / 64; ++w) {
-                    pre += (w < wave) ? s_wave[w] : 0u;
-                    tot += s_wave[w];
-                }
-                if (hit) {
-                    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    const uint32_t dst = lo + run + pre + below;
-                    if (dst < hi) {
-                        sorted_index[dst] = j;
-                        const float4 p = binned_pos(x, v, j, a);
-                        sorted[dst] = p;
-                        store_xyz(xyz, dst, p);
-                        if (slot_of) slot_of[j] = dst;
-                    }
-                }
-                run += tot;
-                __syncthreads();
+        }
+    }
+    if (!*huge_flag) return;
+    // Huge cells: wave W places the members of index block W in index order.
+    // A member's rank in its cell is the rank of the block's first member
+    // (hoff) plus the members of the cell before it in the block, counted
+    // 64 indices at a time: a ballot per huge cell present gives each lane
+    // its group's first lane and its rank in the group, then one atomic
+    // from each group's first lane (all groups at once) moves that cell's
+    // running rank on.
+    static_assert(kBigGrid * (kBigBlock / 64) == kHugeBlocks, "one wave per index block");
+    const uint32_t W = blockIdx.x * (kBigBlock / 64) + wave;
+    const uint32_t R = huge_block_len(n);
+    const uint32_t b0 = W * R, b1 = min(b0 + R, n);
+    for (uint32_t i0 = b0; i0 < b1; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        uint32_t key = 0, lo = 0;
+        bool huge = false;
+        if (i < b1) {
+            key = keys[i];
+            const CellRange r = cell_range(cell_start, key);
+            lo = r.lo;
+            const uint32_t len = r.hi - lo;
+            huge = len > kLdsSortMax && len <= n - lo;
+        }
+        uint64_t todo = __ballot(huge);
+        if (!todo) continue;
+        int first = 0;
+        uint32_t rank = 0, group = 0;
+        while (todo) {
+            const int leader = (int)__builtin_ctzll(todo);
+            const uint32_t hk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+            const uint64_t peers = __ballot(huge && key == hk);
+            if ((peers >> lane) & 1ull) {
+                first = leader;
+                rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1ull));
+            }
+            if ((int)lane == leader) group = (uint32_t)__popcll(peers);
+            todo &= ~peers;
+        }
+        uint32_t base = 0;
+        if (group) base = atomicAdd(hoff + lo + W, group);
+        base = (uint32_t)__shfl((int)base, first);
+        if (huge) {
+            const uint32_t dst = lo + base + rank;
+            if (dst < n) {
+                sorted_index[dst] = i;
+                const float4 p = binned_pos(x, v, i, a);
+                sorted[dst] = p;
+                store_xyz(xyz, dst, p);
+                if (slot_of) slot_of[i] = dst;
             }
         }
     }
@@ -228,7 +305,7 @@ inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 hipError_t launch_predict_hash_count(const HashBufs& b, const StepArgs& a, hipStream_t s) {
     const uint32_t grid = blocks_for(a.n > 0 ? a.n : 1, kBlock);
     hipLaunchKernelGGL(k_predict_hash_count, dim3(grid), dim3(kBlock), 0, s, b.x, b.v, a, b.keys,
-                       b.tickets, b.counts, b.big_count, b.heavy_count);
+                       b.tickets, b.counts, b.big_count, b.heavy_count, b.huge_flag);
     return hipGetLastError();
 }
 
@@ -236,15 +313,16 @@ hipError_t launch_scatter_gather(const HashBufs& b, const StepArgs& a, hipStream
     if (a.n == 0) return hipSuccess;
     const uint32_t grid = blocks_for(a.n, kBlock);
     hipLaunchKernelGGL(k_scatter_ticket, dim3(grid), dim3(kBlock), 0, s, b.keys, b.tickets,
-                       b.cell_start, b.x, b.v, a, b.tmp, b.stage, b.err);
+                       b.cell_start, b.x, b.v, a, b.tmp, b.stage, b.err, b.hcnt);
     hipLaunchKernelGGL(k_rank_gather, dim3(grid), dim3(kBlock), 0, s, b.tmp, b.stage, b.cell_start, a.n,
-                       b.sorted_index, b.sorted, b.big_list, b.big_count, b.err, b.slot_of, b.xyz);
+                       b.sorted_index, b.sorted, b.big_list, b.big_count, b.err, b.slot_of, b.xyz, b.hcnt,
+                       b.hoff, b.huge_flag);
     // Only cells above kSmallCell particles exist past this point; with
     // n <= kSmallCell there can be none.
     if (a.n > kSmallCell)
         hipLaunchKernelGGL(k_big_cells, dim3(kBigGrid), dim3(kBigBlock), 0, s, b.tmp, b.keys,
                            b.cell_start, b.x, b.v, a, b.big_list, b.big_count, b.sorted_index,
-                           b.sorted, b.slot_of, b.xyz);
+                           b.sorted, b.slot_of, b.xyz, b.hoff, b.huge_flag);
     return hipGetLastError();
 }
 

@@ -41,6 +41,16 @@ __device__ __forceinline__ float nbr_key(float rx, float ry, float rz, float h2)
     return __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, __builtin_fmaf(rx, rx, -h2)));
 }
 
+// cell_start[key] and cell_start[key + 1] with one 8-byte load (4-byte
+// aligned: one request per lane where two separate loads made two, which
+// counts where the keys are in random order).
+struct __attribute__((packed, aligned(4))) CellRange {
+    uint32_t lo, hi;
+};
+__device__ __forceinline__ CellRange cell_range(const uint32_t* __restrict__ cell_start, uint32_t key) {
+    return *reinterpret_cast<const CellRange*>(cell_start + key);
+}
+
 // Scene of particle (or sorted slot) i: the last s with p_off[s] <= i.
 // With one scene the result is the constant 0, so scene fields are read
 // with uniform (scalar) loads.

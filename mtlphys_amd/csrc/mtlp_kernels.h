@@ -26,6 +26,16 @@ constexpr uint32_t kSmallCell = 256;      // cells up to this size are ranked in
 constexpr uint32_t kLdsSortMax = 16384;   // cells up to this size: LDS bitonic sort
 constexpr uint32_t kBigBlock = 1024;      // workgroup of the big-cell kernel
 constexpr uint32_t kBigGrid = 128;        // workgroups of the big-cell kernel
+// Cells above kLdsSortMax ("huge") are ranked in index order: the particle
+// indices are cut into kHugeBlocks blocks of ceil(n / kHugeBlocks); the
+// scatter counts each huge cell's members per block into the cell's first
+// kHugeBlocks slots of hcnt, the ranking pass turns those counts into the
+// blocks' starting ranks (hoff), and one wave of the big-cell kernel per
+// block places the block's members in index order.  Linear in n however
+// many huge cells there are.
+constexpr uint32_t kHugeBlocks = kBigGrid * (kBigBlock / 64);  // 2048, one per wave of the big-cell kernel
+static_assert(kHugeBlocks <= kLdsSortMax, "a huge cell holds its block column");
+__host__ __device__ inline uint32_t huge_block_len(uint32_t n) { return (n + kHugeBlocks - 1u) / kHugeBlocks; }
 constexpr uint32_t kChunkSlack = 32;      // readable slots past the end of every sorted position buffer
 // Largest packed (x, y, z) copy (slots incl. slack): its 12-byte slots are
 // addressed with 32-bit byte offsets.  Larger engines run without copies.
@@ -113,6 +123,9 @@ struct HashBufs {
     uint32_t* slot_of;    // sorted slot of each particle (original order); may be NULL
     uint32_t* heavy_count; // 4 heavy-queue counters, cleared by the build; may be NULL
     float* xyz = nullptr;  // packed (x, y, z) of `sorted`, 12 bytes a slot; may be NULL
+    uint32_t* hcnt = nullptr;      // n: huge cells' member counts per index block, zero between builds
+    uint32_t* hoff = nullptr;      // n: their starting ranks per index block
+    uint32_t* huge_flag = nullptr; // set when this build has a huge cell
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------

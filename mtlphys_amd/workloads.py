@@ -11,10 +11,10 @@ differentiable(n) the adjoint workload: the dam break at h = 0.04 (spacing
                   k 1e-4), 3 iterations, 40 steps with checkpoints every 8; loss
                   |centroid(x_T) - target|^2 + 0.1 mean((rho/rho0 - 1)^2) with
                   target = initial centroid + (0.2, 0, 0)
-unit_grid_params, uniform_points, clustered_points
+unit_grid_params, uniform_points, clustered_points, huge_cell_points
                   the isolated spatial hash: 4,194,304 points in the unit
-                  box's 128^3 grid, uniform or with 90% of them in 5% of the
-                  cells
+                  box's 128^3 grid, uniform, with 90% of them in 5% of the
+                  cells, or (a stress case) 95% of them in 200 cells
 ensemble(...)     independent cube drops with N_s = round(exp(U[ln 2048,
                   ln 65536])), rest density U[5000, 7000] (times (0.1/h)^3),
                   gravity U[-12, -8], 3 iterations, drawn from numpy's PCG64
@@ -150,6 +150,23 @@ def clustered_points(n=HASH_POINTS, seed=2, cells=HASH_CELLS, frac_cells=0.05, f
     return np.concatenate([pts, np.ones((n, 1), np.float32)], 1).astype(np.float32)
 
 
+def huge_cell_points(n=HASH_POINTS, seed=2, cells=HASH_CELLS, n_hot=200, frac_pts=0.95):
+    """n points of which frac_pts fall in n_hot cells (about 19,900 each at
+    the defaults: every one above the 16,384 the LDS sort of a cell holds),
+    the rest uniform; FP32, w = 1, in shuffled order.  A stress case for the
+    hash's ranking of huge cells."""
+    rng = np.random.default_rng(seed)
+    hot = rng.permutation(cells ** 3)[:n_hot]
+    k = int(round(frac_pts * n))
+    pick = hot[rng.integers(0, n_hot, k)]
+    cx, cy, cz = pick % cells, (pick // cells) % cells, pick // (cells * cells)
+    off = rng.random((k, 3), dtype=np.float32)
+    hot_pts = (np.stack([cx, cy, cz], 1) + off * np.float32(0.999)) / np.float32(cells)
+    cold_pts = rng.random((n - k, 3), dtype=np.float32)
+    pts = np.concatenate([hot_pts.astype(np.float32), cold_pts])[rng.permutation(n)]
+    return np.concatenate([pts, np.ones((n, 1), np.float32)], 1).astype(np.float32)
+
+
 def hash_bytes(n, cells):
     """Compulsory bytes of hash + count + scan + stable scatter + gather:
     28 N + 16 C."""
@@ -158,4 +175,4 @@ def hash_bytes(n, cells):
 
 __all__ = ["DT", "ADJ_STEPS", "ADJ_CKPT", "ADJ_SHIFT", "differentiable", "dam_break", "cube_drop", "ensemble_config",
            "make_ensemble", "step_bytes", "step_bytes_stored", "HASH_POINTS", "HASH_CELLS", "unit_grid_params",
-           "uniform_points", "clustered_points", "hash_bytes", "Engine", "Batch"]
+           "uniform_points", "clustered_points", "huge_cell_points", "hash_bytes", "Engine", "Batch"]

@@ -189,7 +189,7 @@ def test_hash_4m_128cubed_bitexact(mtlp, orc, variant):
 @pytest.mark.parametrize("n_in_cell", [300, 5000, 16384, 16385, 70000])
 def test_hash_big_cells_stable(mtlp, orc, n_in_cell):
     """Cells above the in-place ranking limit: LDS sort up to 16384 particles,
-    ordered key scan beyond; mixed with ordinary cells."""
+    ranks per index block beyond; mixed with ordinary cells."""
     rng = np.random.default_rng(n_in_cell)
     n_rest = 20000
     hot = np.tile(np.array([[0.5, 0.5, 0.5]], np.float32), (n_in_cell, 1))
@@ -206,6 +206,39 @@ def test_hash_big_cells_stable(mtlp, orc, n_in_cell):
         sidx = e.sorted_index_buffer().read()
         s4 = e.sorted_positions_buffer().read()
         assert e.error_flags() == 0
+    np.testing.assert_array_equal(sidx, np.argsort(keys, kind="stable").astype(np.uint32))
+    np.testing.assert_array_equal(s4.view(np.uint32), pts[sidx].view(np.uint32))
+
+
+def _hash_only_step(mtlp, p, pts):
+    """One step with no solver iteration, no gravity and zero velocities: the
+    spatial hash of pts exactly (x* = x), without countNeighbors, whose
+    stencil walk is quadratic in a cell's size.  Returns (keys, sorted
+    index, sorted positions)."""
+    q = p.copy()
+    q.iterations = 0
+    for a in range(3):
+        q.gravity[a] = 0.0
+    with mtlp.Engine(0, q) as e:
+        e.reset(len(pts), mtlp.capi.SCENE_EMPTY)
+        e.upload_state(pts, np.zeros_like(pts))
+        e.step(1.0 / 120.0)
+        out = (e.cell_hashes_buffer().read(), e.sorted_index_buffer().read(), e.sorted_positions_buffer().read())
+        assert e.error_flags() == 0
+    return out
+
+
+@pytest.mark.parametrize("n,hot", [(400000, 20), (4194304, 200)])
+def test_hash_many_huge_cells_stable(mtlp, n, hot):
+    """95% of the points in `hot` cells of ~19,000 each (all above the
+    16,384 the LDS sort holds), the rest uniform: every huge cell is ranked
+    per index block, together in one pass over the indices, and the
+    permutation must be the stable argsort of the keys."""
+    from mtlphys_amd import workloads as W
+    pts = W.huge_cell_points(n, 7, n_hot=hot)
+    keys, sidx, s4 = _hash_only_step(mtlp, _unit_grid_params(mtlp), pts)
+    counts = np.bincount(keys)
+    assert (counts > 16384).sum() == hot, np.sort(counts)[-hot - 1:]
     np.testing.assert_array_equal(sidx, np.argsort(keys, kind="stable").astype(np.uint32))
     np.testing.assert_array_equal(s4.view(np.uint32), pts[sidx].view(np.uint32))
 
