@@ -353,9 +353,9 @@ mtlp_status enqueue_step(mtlp_engine* e, hipStream_t s, float dt) {
         HIP_TRY(launch_count_neighbors(e->sorted[0], e->cell_start, e->sorted_index, a, e->nbr, s));
         clk.mark("count_neighbors");
     }
-    // lambda of the last density pass is in .w of the other iterate buffer
-    HIP_TRY(launch_finalize(e->sorted[cur], e->slot_of, a, e->x, e->v, e->prev, e->nbr, e->foam,
-                            iters ? e->sorted[cur ^ 1] : nullptr, e->lambdas, s));
+    // the last delta pass left each slot's lambda in .w of the final iterate
+    HIP_TRY(launch_finalize(e->sorted[cur], e->slot_of, a, e->x, e->v, e->prev, e->nbr, e->foam, iters > 0,
+                            e->lambdas, s));
     clk.mark("scatter_finalize");
     e->final_buf = cur;
     return MTLP_OK;

@@ -278,11 +278,12 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
                               float4* next, const NbrList& nl, uint32_t block, hipStream_t s);
 // Scatter back + finalize in original order: particle i takes its result
 // from slot slot_of[i]; with foam non-NULL also computeFoam from the step's
-// neighbour counts (nbr, original order); with lam_sorted (the last density
-// iterate, lambda in .w) it also writes lambda_orig.
+// neighbour counts (nbr, original order); with lam_in_w (a step with
+// solver iterations: the last delta pass leaves each slot's lambda in .w of
+// the final iterate) it also writes lambda_orig.
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
-                           const uint32_t* nbr, float* foam, const float4* lam_sorted, float* lambda_orig,
+                           const uint32_t* nbr, float* foam, bool lam_in_w, float* lambda_orig,
                            hipStream_t s);
 // integrate (Integrate.metal:11-42) on n particles; bounds from b.
 hipError_t launch_integrate(float4* x, float4* v, uint32_t n, float dt, float gy, float damping,

@@ -632,6 +632,7 @@ __device__ __forceinline__ void finish_delta(uint32_t s, const float4& pi, const
                            (float)fmin(fmax((double)pi.y + dy, (double)sd.bminy), (double)sd.bmaxy),
                            (float)fmin(fmax((double)pi.z + dz, (double)sd.bminz), (double)sd.bmaxz), 1.0f);
     if (sphere.on) q = sphere_push(q, sphere);
+    q.w = pi.w;  // lambda_s, which this iterate was computed with (read by finalize)
     next[s] = q;
     store_xyz(xyz_next, s, q);
 }

@@ -74,23 +74,25 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(5))) void k_
 
 // Scatter back + finalize (+ computeFoam), in original order: particle i
 // reads its final iterate at slot slot_of[i]; its old position (still in x)
-// becomes prev; x and v get the step's result.  With lam_sorted, lambda of
-// the last density pass (in .w of that iterate) goes to lambda_orig[i].
+// becomes prev; x and v get the step's result.  With lam_in_w, lambda of
+// the last density pass (which the delta pass left in .w of the final
+// iterate) goes to lambda_orig[i].
 __global__ __launch_bounds__(kBlock) void k_finalize(
     const float4* __restrict__ fin, const uint32_t* __restrict__ slot_of, StepArgs a,
     float4* __restrict__ x, float4* __restrict__ v, float4* __restrict__ prev, const uint32_t* __restrict__ nbr,
-    float* __restrict__ foam, const float4* __restrict__ lam_sorted, float* __restrict__ lambda_orig) {
+    float* __restrict__ foam, bool lam_in_w, float* __restrict__ lambda_orig) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
     with_scene(i, a, [&](const SceneDev& sc) {
         const uint32_t s = slot_of[i];
         const float4 old = x[i];
         prev[i] = old;
-        const Finalized r = finalize_particle(sc.fd, a.dt, fin[s], old, foam != nullptr, foam ? nbr[i] : 0u);
+        const float4 pf = fin[s];
+        const Finalized r = finalize_particle(sc.fd, a.dt, pf, old, foam != nullptr, foam ? nbr[i] : 0u);
         x[i] = r.x;
         v[i] = r.v;
         if (foam) foam[i] = r.foam;
-        if (lam_sorted) lambda_orig[i] = lam_sorted[s].w;
+        if (lam_in_w) lambda_orig[i] = pf.w;
     });
 }
 
@@ -217,11 +219,11 @@ hipError_t launch_apply_delta(const float4* cur, const uint32_t* cell_start, con
 
 hipError_t launch_finalize(const float4* final_sorted, const uint32_t* slot_of,
                            const StepArgs& a, float4* x, float4* v, float4* prev,
-                           const uint32_t* nbr, float* foam, const float4* lam_sorted, float* lambda_orig,
+                           const uint32_t* nbr, float* foam, bool lam_in_w, float* lambda_orig,
                            hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(blocks_for(a.n, kBlock)), dim3(kBlock), 0, s, final_sorted,
-                       slot_of, a, x, v, prev, nbr, foam, lam_sorted, lambda_orig);
+                       slot_of, a, x, v, prev, nbr, foam, lam_in_w, lambda_orig);
     return hipGetLastError();
 }
 
