@@ -252,7 +252,9 @@ mtlp_status mtlp_set_graph(mtlp_engine* e, int enable);
  * (workgroup size 64, 128, 256 or 512 times tiles per pull 0, 1 or 4, see
  * mtlp_set_tile_pull, times packed positions on or off, see
  * mtlp_set_packed_positions), two passes over them, from the current state,
- * and keeps the configuration with the fastest pass.  Afterwards every buffer a
+ * and keeps the configuration with the fastest pass, unless the engine's
+ * current configuration is within 3% of it (then that one stays).
+ * Afterwards every buffer a
  * caller can read (positions, velocities, previous positions, cell hashes,
  * cell offsets, sorted index and positions, neighbour counts, lambdas,
  * foam) holds what it held before the call.  With cache_path, reads a

@@ -876,9 +876,21 @@ mtlp_status mtlp_autotune(mtlp_engine* e, mtlp_stream stream, float dt, const ch
         }
         if (st == MTLP_OK) cfg_ms[ci] = std::min(cfg_ms[ci], ms);
     }
+    // The configuration the engine ran with stays unless another is more
+    // than kTuneMargin faster: neighbouring workgroup sizes are within a few
+    // per cent of each other on the dam break, and short timings flip them.
+    constexpr float kTuneMargin = 0.03f;
     uint32_t best = 0;
     for (uint32_t ci = 1; ci < configs; ++ci)
         if (cfg_ms[ci] < cfg_ms[best]) best = ci;
+    uint32_t incumbent = configs;
+    for (uint32_t ci = 0; ci < configs; ++ci) {
+        const uint32_t li = ci % kLaunches;
+        if (kStencilBlocks[li / kNumTilePulls] == old_block && kTilePulls[li % kNumTilePulls] == old_pull &&
+            (packs == 2u && ci < kLaunches) == old_packed)
+            incumbent = ci;
+    }
+    if (incumbent < configs && cfg_ms[best] > cfg_ms[incumbent] * (1.0f - kTuneMargin)) best = incumbent;
     const hipError_t restore = copy_parts(false);
     const hipError_t sync = hipStreamSynchronize(s);
     e->final_buf = final_buf;
