@@ -270,3 +270,43 @@ def test_ensemble_scenes_kernel_level_vs_fp64(mtlp, orc):
         xe, ve = position_error_h(fin["x"], xr, h), velocity_error(fin["v"], vr, h, DT)
         print(f"scene {s} finalize: x {xe['max']:.2e} h v {ve['max']:.2e} h/dt", flush=True)
         assert xe["max"] <= X_TOL_H and ve["max"] <= V_TOL, (s, xe, ve)
+
+
+def test_batch_huge_cells_stable(mtlp):
+    """Three scenes in one batch, two of them with cells far above the 16,384
+    particles an LDS sort holds, whose members are ranked per block of the
+    batch's particle indices: the permutation must be the stable argsort of
+    the (global) cell keys, so every scene keeps its particles in index
+    order within a cell.  Steps with no solver iteration and zero gravity
+    and velocity: the hash of the positions as they are."""
+    from mtlphys_amd import workloads as W
+    p = W.unit_grid_params(64)
+    p.iterations = 0
+    for a in range(3):
+        p.gravity[a] = 0.0
+    rng = np.random.default_rng(31)
+
+    def scene(n, hot):
+        pts = [rng.random((n - sum(hot), 3), dtype=np.float32)]
+        for m in hot:
+            c = rng.integers(0, 64, 3)
+            pts.append(((c + rng.random((m, 3)) * 0.999) / 64.0).astype(np.float32))
+        x = np.concatenate(pts)[rng.permutation(n)]
+        return np.concatenate([x, np.ones((n, 1), np.float32)], 1)
+
+    counts = [40000, 8000, 30000]
+    xs = [scene(40000, [17500, 17500]), scene(8000, []), scene(30000, [20000])]
+    x = np.concatenate(xs).astype(np.float32)
+    b = mtlp.Batch(counts, [p, p, p])
+    try:
+        b.upload_state(x, np.zeros_like(x))
+        b.step(DT)
+        keys = b.cell_hashes_buffer().read()
+        sidx = b.sorted_index_buffer().read()
+        s4 = b.sorted_positions_buffer().read()
+        assert b.error_flags() == 0
+    finally:
+        b.close()
+    assert (np.bincount(keys) > 16384).sum() == 3
+    np.testing.assert_array_equal(sidx, np.argsort(keys, kind="stable").astype(np.uint32))
+    np.testing.assert_array_equal(s4.view(np.uint32), x[sidx].view(np.uint32))
