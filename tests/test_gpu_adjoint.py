@@ -416,14 +416,17 @@ def test_differentiable_workload_full_length(mtlp, orc):
     assert max(fd_errs) < 1e-3, max(fd_errs)
 
 
-def _remainder_pairs(mtlp, e, steps, p):
+def _remainder_pairs(mtlp, e, steps, p, per_slot_max=False):
     """Pairs (s, c) of the taped steps' passes with c in N_s (c's bin in the
     stencil of s's current cell, |r| < h) but bin(s) outside the stencil of
     c's current cell: the pairs whose share for c goes to c's inbox.  From
-    the float64 neighbour pairs of a k-d tree (a count, not a parity check)."""
+    the float64 neighbour pairs of a k-d tree (a count, not a parity check).
+    With per_slot_max, also the most such pairs one slot receives in one
+    iteration."""
     from scipy.spatial import cKDTree
     C = mtlp.capi
     dims = np.array(list(C.grid_dims(p)), np.int64)
+    most = 0
     lo = np.array(p.bounds_min, np.float32)
     inv = np.float32(1.0) / np.float32(p.cell_size)
     total = 0
@@ -437,8 +440,11 @@ def _remainder_pairs(mtlp, e, steps, p):
             i = np.concatenate([pr[:, 0], pr[:, 1]])
             j = np.concatenate([pr[:, 1], pr[:, 0]])
             listed = (np.abs(b[j] - c[i]) <= 1).all(1)
-            total += int((listed & ~(np.abs(b[i] - c[j]) <= 1).all(1)).sum())
-    return total
+            rem = listed & ~(np.abs(b[i] - c[j]) <= 1).all(1)
+            total += int(rem.sum())
+            if rem.any():
+                most = max(most, int(np.bincount(j[rem]).max()))
+    return (total, most) if per_slot_max else total
 
 
 def test_adjoint_deterministic_with_remainder_terms(mtlp):
@@ -476,3 +482,48 @@ def test_adjoint_deterministic_with_remainder_terms(mtlp):
     assert pairs > 1000, pairs
     for a, b in zip(outs[0], outs[1]):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_step_vjp_dense_blob_inbox_overflow(mtlp, orc):
+    """A blob packed at ~100 particles per h^3 inside a cube drop: in its
+    first step the solver pushes the blob's particles the full 0.3 h cap per
+    iteration, and some slots receive more remainder pair terms in one pass
+    than their inbox holds (two per pair: delta and density passes), so the
+    overflow path (float atomics) runs beside the inboxes.  The one-step
+    gradient must still match the FP64 oracle adjoint on the GPU tape
+    within 1e-4 relative (norm) per output."""
+    import torch
+    p = mtlp.scene_params(3, 8192)
+    x, v = mtlp.generate_scene(3, 8192, 5, p)
+    rng = np.random.default_rng(13)
+    centre = x[:, :3].mean(axis=0)
+    m = 3000
+    side = (m / 100.0) ** (1.0 / 3.0)
+    x[:m, :3] = (centre + rng.uniform(-0.5, 0.5, (m, 3)) * side * p.h).astype(np.float32)
+    n = len(x)
+    xbar = np.zeros((n, 4), np.float32)
+    vbar = np.zeros((n, 4), np.float32)
+    xbar[:, :3] = rng.standard_normal((n, 3))
+    vbar[:, :3] = rng.standard_normal((n, 3))
+    dev = torch.device("cuda")
+    with _engine(mtlp, p, x, v) as e:
+        s = torch.cuda.current_stream()
+        e.rollout(DT, 1, 1, s)
+        xo, vo, go = (torch.empty((n, 4), device=dev), torch.empty((n, 4), device=dev),
+                      torch.empty(3, device=dev))
+        e.adjoint_rollout(DT, 1, 1, torch.from_numpy(xbar).to(dev), torch.from_numpy(vbar).to(dev), xo, vo, go, s)
+        torch.cuda.synchronize()
+        gx, gv, gg = xo.cpu().numpy(), vo.cpu().numpy(), go.cpu().numpy()
+        e.upload_state(x, v)
+        e.tape_record(DT, 1)
+        tape = _cpu_tape(mtlp, e, 0, p.iterations)
+        pairs, most = _remainder_pairs(mtlp, e, 1, p, per_slot_max=True)
+        assert e.error_flags() == 0
+    print("remainder pairs", pairs, "most in one slot and pass", most, flush=True)
+    assert 2 * most > 32, most  # the inbox of that slot overflowed
+    op = orc.params(p)
+    rx, rv, rg = orc.step_vjp_d(tape, op, DT, xbar.astype(np.float64), vbar.astype(np.float64),
+                                orc.adjoint_sums(xbar, vbar))
+    errs = dict(x=_rel(gx, rx), v=_rel(gv, rv), g=_rel(gg, rg))
+    print("dense blob", errs, flush=True)
+    assert errs["x"] < 1e-4 and errs["v"] < 1e-4 and errs["g"] < 1e-4, errs
