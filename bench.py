@@ -12,7 +12,8 @@ One JSON line on stdout:
                 B_step = (100 + 44 I) N + (16 + 8 I) C bytes (the compulsory
                 traffic of every stage: vec3 = 12 B, u32/f32 = 4 B), over the
                 K timed steps after W warm-up steps (steps W+1 .. W+K)
-  ms_per_step   wall time of those K steps between two device
+  ms_per_step   wall time of those K steps (each replayed from the engine's
+                HIP graph, mtlp_set_graph) between two device
                 synchronisations / K
   ms_per_step_late, value_late
                 the same workload over steps 81-120 (after the column hits
@@ -279,6 +280,7 @@ def step_curve(steps, torch, window=5):
     eng.upload_state(x, v)
     eng.set_stencil_block(64)
     stream = torch.cuda.current_stream()
+    eng.set_graph(True)
     out = []
     for _ in range(max(1, steps // window)):
         torch.cuda.synchronize()
@@ -289,6 +291,7 @@ def step_curve(steps, torch, window=5):
         out.append(round((time.perf_counter() - t0) / window * 1e3, 3))
     counts = eng.neighbor_counts_buffer().read()
     # one more step with the engine's per-kernel events: us per stage
+    eng.set_graph(False)
     eng.set_profiling(True)
     eng.step(DT, stream)
     stages = {}
@@ -549,8 +552,10 @@ def main():
     eng.set_profiling(False)
 
     # the same K steps (same initial state and warm-up) without events or
-    # host waits between steps
+    # host waits between steps, each replayed from the engine's HIP graph
+    # (captured during the warm-up)
     eng.upload_state(x, v)
+    eng.set_graph(True)
     for _ in range(args.warmup):
         eng.step(DT, stream)
     torch.cuda.synchronize()
