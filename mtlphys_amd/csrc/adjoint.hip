@@ -218,9 +218,10 @@ __device__ __forceinline__ void remainder_put(uint32_t* __restrict__ rcnt, float
 }
 
 // Order-independent sums of components kFirst .. kFirst + kC - 1 of the
-// m <= kInbox entries e[0..m): every FP32 value of a component is scaled by 2^S to a 64-bit integer, S set by the largest
-// exponent in the component (|total| < 2^61), the integers are added and the
-// total is rounded back to FP32 once.  Any arrival order gives the same bits;
+// m <= kInbox entries e[0..m): every FP32 value of a component is scaled by
+// 2^S to a 64-bit integer, S set by the largest exponent in the component
+// (each term below 2^57 in magnitude, so |total| < 2^62 for 32 entries), the
+// integers are added and the total is rounded back to FP32 once.  Any arrival order gives the same bits;
 // parts below 2^-57 of the largest value are truncated toward zero.  A
 // component holding an infinity or NaN is summed as floats (the result is
 // not finite either way).

@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
             const CellRange r = cell_range(cell_start, key);
             lo = r.lo;
             const uint32_t len = r.hi - lo;
-            huge = len > kLdsSortMax && len <= n - lo;
+            huge = lo <= n && len > kLdsSortMax && len <= n - lo;  // hi <= n, no wrap
         }
         uint64_t todo = __ballot(huge);
         if (!todo) continue;
