@@ -140,6 +140,16 @@ __device__ __forceinline__ void store_xyz(float* __restrict__ xyz, uint32_t s, c
     q[2] = p.z;
 }
 
+// (x, y, z, 0) of slot s of a packed copy: one 12-byte load (4-byte aligned)
+// where the float4 iterate takes 16 bytes.
+struct __attribute__((packed, aligned(4))) Words3 {
+    float a, b, c;
+};
+__device__ __forceinline__ float4 load_xyz(const float* __restrict__ xyz, uint32_t s) {
+    const Words3 w = *reinterpret_cast<const Words3*>(reinterpret_cast<const char*>(xyz) + s * 12u);
+    return make_float4(w.a, w.b, w.c, 0.0f);
+}
+
 // Inclusive wave64 prefix sum with DPP: row_shr 1/2/4/8 inside each 16-lane
 // row, then row_bcast:15 and row_bcast:31 carry row totals forward.  Lanes
 // whose DPP source is invalid or whose row is masked off read `old` = 0.
