@@ -289,6 +289,22 @@ def test_step_kernel_level_after_settling(mtlp, orc):
     _kernel_level(mtlp, orc, p, x1, v1, 1.0 / 120.0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 257, 4097])
+def test_step_ragged_sizes(mtlp, orc, n):
+    """Partial waves, workgroups and tiles: 1 and 2 particles, one short of
+    and one past a wave, one past a workgroup and one past a 4,096-slot
+    tile.  Integer stages bit-exact against the oracle, then every
+    density/lambda and delta kernel and finalize against the FP64 oracle."""
+    p, x, v = _cube_case(mtlp, n=n, seed=5)
+    dt = 1.0 / 120.0
+    got = run_gpu_step(mtlp, p, x, v, dt)
+    _, _, tr = orc.step(x, v, orc.params(p), dt, prec=32, trace=True)
+    for k in ("keys", "cell_start", "sorted_index"):
+        np.testing.assert_array_equal(got[k], tr[k], err_msg=k)
+    np.testing.assert_array_equal(got["nbr"], tr["nbr_count"])
+    _kernel_level(mtlp, orc, p, x, v, dt)
+
+
 def _chunks_and_counts(orc, params, x, v, dt):
     """Per particle at iteration 0: the stencil chunks the density pass keeps
     (chunks of <= 32 candidate slots of a stencil row, as it cuts them, that
