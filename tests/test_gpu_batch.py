@@ -112,6 +112,35 @@ def test_graph_replay_matches_eager(mtlp):
         assert eager.error_flags() == 0 and graphed.error_flags() == 0
 
 
+def test_graph_replay_after_parameter_changes(mtlp):
+    """New parameters between replays (gravity and rest density, then the
+    iteration count, which changes the captured kernel sequence) drop the
+    graph: the replayed steps stay bit-identical to eager steps."""
+    p = mtlp.scene_params(3, 4000)
+    x, v = mtlp.generate_scene(3, 4000, 4, p)
+    with mtlp.Engine(0, p) as eager, mtlp.Engine(0, p) as graphed:
+        for e in (eager, graphed):
+            e.reset(4000, mtlp.capi.SCENE_EMPTY)
+            e.upload_state(x, v)
+        graphed.set_graph(True)
+        q = mtlp.scene_params(3, 4000)
+        q.gravity[0], q.gravity[1] = 0.5, -4.0
+        q.rest_density = p.rest_density * 1.1
+        r = mtlp.scene_params(3, 4000)
+        r.iterations = 2
+        for k, params in enumerate((None, q, None, r, None)):
+            if params is not None:
+                for e in (eager, graphed):
+                    e.params = params
+            eager.step(DT)
+            graphed.step(DT)
+            for name in ("positions_buffer", "velocities_buffer", "lambdas_buffer"):
+                a = getattr(eager, name)().read()
+                g = getattr(graphed, name)().read()
+                np.testing.assert_array_equal(a.view(np.uint32), g.view(np.uint32), err_msg=f"step {k} {name}")
+        assert eager.error_flags() == 0 and graphed.error_flags() == 0
+
+
 def test_stencil_block_sizes_bitwise_and_autotune(mtlp, tmp_path):
     p, x, v = None, None, None
     p = mtlp.scene_params(3, 20000)
