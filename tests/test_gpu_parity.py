@@ -289,6 +289,31 @@ def test_step_kernel_level_after_settling(mtlp, orc):
     _kernel_level(mtlp, orc, p, x1, v1, 1.0 / 120.0)
 
 
+def test_step_particles_on_and_past_the_walls(mtlp, orc):
+    """Particles exactly on the box faces and up to h/2 beyond them (binned
+    into the edge cells by the clamp of cellCoordOf, pulled back by the
+    delta pass's box clamp): integer stages bit-exact, every kernel against
+    the FP64 oracle."""
+    p, x, v = _cube_case(mtlp, n=4000, seed=9)
+    lo, hi, h = np.array(p.bounds_min), np.array(p.bounds_max), p.h
+    rng = np.random.default_rng(9)
+    k = len(x) // 10
+    idx = rng.permutation(len(x))
+    past = idx[:k]              # beyond the +x face by up to h/2
+    x[past, 0] = hi[0] + rng.uniform(0.0, 0.5 * h, k).astype(np.float32)
+    on = idx[k:2 * k]           # exactly on the +x face
+    x[on, 0] = np.float32(hi[0])
+    floor = idx[2 * k:3 * k]    # exactly on the floor
+    x[floor, 1] = np.float32(lo[1])
+    dt = 1.0 / 120.0
+    got = run_gpu_step(mtlp, p, x, v, dt)
+    _, _, tr = orc.step(x, v, orc.params(p), dt, prec=32, trace=True)
+    for name in ("keys", "cell_start", "sorted_index"):
+        np.testing.assert_array_equal(got[name], tr[name], err_msg=name)
+    np.testing.assert_array_equal(got["nbr"], tr["nbr_count"])
+    _kernel_level(mtlp, orc, p, x, v, dt)
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 65, 257, 4097])
 def test_step_ragged_sizes(mtlp, orc, n):
     """Partial waves, workgroups and tiles: 1 and 2 particles, one short of
