@@ -102,14 +102,11 @@ def test_rollout_leaves_the_engine_as_steps_do(mtlp):
             assert a.error_flags() == 0 and b.error_flags() == 0
 
 
-@pytest.mark.parametrize("scene", ["cube_bench", "cube_reference_constants", "dam_walls"])
-def test_step_vjp_matches_fp64_adjoint(mtlp, orc, scene):
+def _step_vjp_check(mtlp, orc, p, x, v, label):
     """One step: GPU (xbar_in, vbar_in, gbar) vs the FP64 oracle adjoint on
     the GPU tape.  Tolerance: 1e-4 relative (norm) per output, 1e-3 along
     each of 32 seeded unit directions relative to the gradient's norm."""
     import torch
-    name, p, x, v = next(s for s in _scenes(mtlp) if s[0] == scene)
-    x, v = _warm(mtlp, p, x, v, 10)
     n = len(x)
     rng = np.random.default_rng(11)
     xbar = np.zeros((n, 4), np.float32)
@@ -134,7 +131,7 @@ def test_step_vjp_matches_fp64_adjoint(mtlp, orc, scene):
     rx, rv, rg = orc.step_vjp_d(tape, op, DT, xbar.astype(np.float64), vbar.astype(np.float64),
                                 orc.adjoint_sums(xbar, vbar))
     errs = dict(x=_rel(gx, rx), v=_rel(gv, rv), g=_rel(gg, rg))
-    print(scene, errs, "|xbar_in|", np.linalg.norm(rx))
+    print(label, errs, "|xbar_in|", np.linalg.norm(rx))
     assert np.all(gx[:, 3] == 0) and np.all(gv[:, 3] == 0)
     assert errs["x"] < 1e-4 and errs["v"] < 1e-4 and errs["g"] < 1e-4, errs
     for _ in range(32):
@@ -142,6 +139,25 @@ def test_step_vjp_matches_fp64_adjoint(mtlp, orc, scene):
         d[:, 3] = 0
         d /= np.linalg.norm(d)
         assert abs(((gv - rv) * d).sum()) < 1e-3 * np.linalg.norm(rv)
+        assert abs(((gx - rx) * d).sum()) < 1e-3 * np.linalg.norm(rx)
+
+
+@pytest.mark.parametrize("scene", ["cube_bench", "cube_reference_constants", "dam_walls"])
+def test_step_vjp_matches_fp64_adjoint(mtlp, orc, scene):
+    """One step of a warmed scene against the FP64 adjoint (_step_vjp_check)."""
+    name, p, x, v = next(s for s in _scenes(mtlp) if s[0] == scene)
+    x, v = _warm(mtlp, p, x, v, 10)
+    _step_vjp_check(mtlp, orc, p, x, v, scene)
+
+
+@pytest.mark.parametrize("n", [1, 65, 257])
+def test_step_vjp_ragged_sizes(mtlp, orc, n):
+    """The backward kernels run four lanes per slot: partial waves and
+    workgroups at 1, 65 and 257 particles, against the FP64 adjoint."""
+    p = mtlp.scene_params(3, n)
+    x, v = mtlp.generate_scene(3, n, 5, p)
+    x, v = _warm(mtlp, p, x, v, 5)
+    _step_vjp_check(mtlp, orc, p, x, v, f"cube_{n}")
 
 
 def test_rollout_checkpointing(mtlp, orc):
