@@ -190,7 +190,7 @@ mtlp_status mtlp_exclusive_scan(mtlp_engine* e, mtlp_stream stream, const uint32
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): mtlp_exclusive_scan without d_scratch and without an
  * engine.  Its working memory is a fixed array in the library's device
- * memory (512 KiB per GPU the library runs on, held until the process
+ * memory (128 KiB per GPU the library runs on, held until the process
  * ends), cleared by a kernel at the start of every call; the call allocates
  * nothing.  n <= MTLP_NOALLOC_SCAN_MAX, else MTLP_E_RANGE; d_in and d_out as
  * for mtlp_exclusive_scan.  Every call on one GPU shares that array: calls

@@ -19,9 +19,11 @@
 namespace mtlp {
 
 constexpr uint32_t kBlock = 256;          // particle-kernel workgroup (4 waves)
-constexpr uint32_t kScanBlock = 256;      // scan workgroup
-constexpr uint32_t kScanItems = 16;       // elements per scan thread (4 x uint4)
-constexpr uint32_t kScanTile = kScanBlock * kScanItems;  // 4096
+constexpr uint32_t kScanBlock = 512;      // scan workgroup (8 waves)
+constexpr uint32_t kScanItems = 32;       // elements per scan thread (8 x uint4) ...
+constexpr uint32_t kScanItemsLarge = 64;  // ... and from kScanLargeFrom elements on (16 x uint4)
+constexpr uint32_t kScanLargeFrom = 1u << 22;
+constexpr uint32_t kScanTile = kScanBlock * kScanItems;  // 16384, the smaller tile (sizes the look-back words)
 constexpr uint32_t kSmallCell = 256;      // cells up to this size are ranked in place
 constexpr uint32_t kLdsSortMax = 16384;   // cells up to this size: LDS bitonic sort
 constexpr uint32_t kBigBlock = 1024;      // workgroup of the big-cell kernel

@@ -5,11 +5,12 @@
 // Engine.cpp:625-648 and :843-882), which is capped at 1024^2 elements
 // (Engine.hpp:93).  Here one launch scans any length below 2^31:
 //
-//   * a tile is 4096 elements: 256 threads x 4 coalesced uint4 loads;
+//   * a tile is 512 threads x 8 coalesced uint4 loads (16,384 elements),
+//     from 2^22 elements on 512 x 16 (32,768);
 //   * inside a thread, 4 consecutive elements are summed serially;
 //   * inside a wave64, thread sums are scanned with DPP row shifts and row
 //     broadcasts (wave_incl_scan), replacing simd_prefix_exclusive_sum;
-//   * the four wave totals of each 1024-element sub-tile go through LDS;
+//   * the eight wave totals of each 2048-element column go through LDS;
 //   * tiles chain by decoupled look-back (Merrill & Garland 2016): tile ids
 //     come from an atomic ticket so a tile only ever waits on tiles that
 //     already run; wave 0 inspects 64 predecessors per step.
@@ -55,12 +56,19 @@ __device__ __forceinline__ void store4(uint32_t* out, size_t e, size_t n, uint4 
     if (e + 2 < n) out[e + 2] = v.z;
 }
 
-__global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(
+// One tile of kB x kItems elements per workgroup: thread t holds uint4
+// column k (k < kItems / 4) at tile offset 4 (k kB + t).
+template <uint32_t kB, uint32_t kItems>
+__global__ __launch_bounds__(kB) void k_exclusive_scan(
     uint32_t* in, uint32_t* out, uint32_t n, uint32_t zero_input,
     uint64_t* __restrict__ state, uint32_t* __restrict__ ticket, uint32_t epoch,
     uint32_t num_tiles, uint32_t* __restrict__ err) {
+    constexpr int kVec = (int)(kItems / 4);  // uint4 per thread
+    constexpr uint32_t kWaves = kB / 64;
+    constexpr size_t kTile = (size_t)kB * kItems;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kScanBlock / 64][4];
+    __shared__ uint32_t s_wave[kWaves][kVec];  // wave totals per column, then their exclusive prefix
+    __shared__ uint32_t s_col[kVec];           // column totals
     __shared__ uint32_t s_excl;
 
     const uint32_t t = threadIdx.x;
@@ -78,38 +86,62 @@ __global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(
     }
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * kScanTile;
+    const size_t base = (size_t)tile * kTile;
 
-    uint4 v[4];
-    uint32_t tsum[4], wincl[4];
+    // The loads go out together before any is used, then the input is
+    // zeroed: a zeroing store between them would hold each later load back
+    // (the two may alias as far as the compiler knows), one memory latency
+    // per load.  A full tile (the test is uniform over the workgroup) takes
+    // plain 16-byte accesses, the last tile the bounds-checked ones.
+    const bool full = base + kTile <= (size_t)n;
+    uint4 v[kVec];
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const size_t e = base + (size_t)k * (kScanBlock * 4) + 4u * t;
-        v[k] = load4(in, e, n);
-        if (zero_input) store4(in, e, n, make_uint4(0u, 0u, 0u, 0u));
-        tsum[k] = v[k].x + v[k].y + v[k].z + v[k].w;
-        wincl[k] = wave_incl_scan(tsum[k]);
-        if (lane == 63) s_wave[wave][k] = wincl[k];
+        for (int k = 0; k < kVec; ++k)
+            v[k] = *reinterpret_cast<const uint4*>(in + base + (size_t)k * (kB * 4) + 4u * t);
+        if (zero_input) {
+#pragma unroll
+            for (int k = 0; k < kVec; ++k)
+                *reinterpret_cast<uint4*>(in + base + (size_t)k * (kB * 4) + 4u * t) =
+                    make_uint4(0u, 0u, 0u, 0u);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kVec; ++k) v[k] = load4(in, base + (size_t)k * (kB * 4) + 4u * t, n);
+        if (zero_input) {
+#pragma unroll
+            for (int k = 0; k < kVec; ++k)
+                store4(in, base + (size_t)k * (kB * 4) + 4u * t, n, make_uint4(0u, 0u, 0u, 0u));
+        }
+    }
+    // per column: the thread's exclusive prefix inside its wave, and the
+    // wave's total into LDS (u32 sums wrap, so any grouping gives the same
+    // result)
+    uint32_t wex[kVec];
+#pragma unroll
+    for (int k = 0; k < kVec; ++k) {
+        const uint32_t ts = v[k].x + v[k].y + v[k].z + v[k].w;
+        const uint32_t wi = wave_incl_scan(ts);
+        wex[k] = wi - ts;
+        if (lane == 63) s_wave[wave][k] = wi;
+    }
+    __syncthreads();
+    if (t < (uint32_t)kVec) {  // column t: exclusive prefix over the waves, and its total
+        uint32_t run = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; ++w) {
+            const uint32_t x = s_wave[w][t];
+            s_wave[w][t] = run;
+            run += x;
+        }
+        s_col[t] = run;
     }
     __syncthreads();
 
-    uint32_t wprefix[4], subtot[4];
-    uint32_t agg = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        uint32_t pre = 0, tot = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < kScanBlock / 64; ++w) {
-            const uint32_t x = s_wave[w][k];
-            pre += (w < wave) ? x : 0u;
-            tot += x;
-        }
-        wprefix[k] = pre;
-        subtot[k] = tot;
-        agg += tot;
-    }
-
     if (wave == 0) {
+        uint32_t agg = 0;
+#pragma unroll
+        for (int k = 0; k < kVec; ++k) agg += s_col[k];
         uint32_t excl = 0;
         if (tile == 0) {
             if (lane == 0)
@@ -154,15 +186,16 @@ __global__ __launch_bounds__(kScanBlock) void k_exclusive_scan(
 
     uint32_t run = s_excl;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const size_t e = base + (size_t)k * (kScanBlock * 4) + 4u * t;
+    for (int k = 0; k < kVec; ++k) {
+        const size_t e = base + (size_t)k * (kB * 4) + 4u * t;
         uint4 o;
-        o.x = run + wprefix[k] + (wincl[k] - tsum[k]);
+        o.x = run + s_wave[wave][k] + wex[k];
         o.y = o.x + v[k].x;
         o.z = o.y + v[k].y;
         o.w = o.z + v[k].z;
-        store4(out, e, n, o);
-        run += subtot[k];
+        if (full) *reinterpret_cast<uint4*>(out + e) = o;
+        else store4(out, e, n, o);
+        run += s_col[k];
     }
 }
 
@@ -179,7 +212,7 @@ uint32_t scan_tiles(uint32_t n) { return (uint32_t)(((uint64_t)n + kScanTile - 1
 // Working memory of the allocation-free scan (mtlp_exclusive_scan_noalloc):
 // 4 u32 words (tile ticket, error flag, 2 unused), then one look-back word
 // per tile of the longest scan it accepts, kNoAllocScanMax elements
-// (65,536 tiles: 512 KiB + 16 bytes).  Each GPU the library runs on holds
+// (16,384 tiles: 128 KiB + 16 bytes).  Each GPU the library runs on holds
 // its own copy of this array until the process ends, and every call on a
 // GPU shares it.
 static __device__ __align__(256) uint64_t scratch[2 + kNoAllocScanMax / kScanTile];
@@ -211,9 +244,21 @@ hipError_t launch_exclusive_scan(uint32_t* in, uint32_t* out, uint32_t n, bool z
                                  uint64_t* state, uint32_t* ticket, uint32_t epoch,
                                  uint32_t* err, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const uint32_t tiles = scan_tiles(n);
-    hipLaunchKernelGGL(k_exclusive_scan, dim3(tiles), dim3(kScanBlock), 0, s, in, out, n,
-                       zero_input ? 1u : 0u, state, ticket, epoch, tiles, err);
+    // Tiles twice as long from kScanLargeFrom on: the inclusive prefixes
+    // advance by one look-back window (64 tiles) per poll, so long scans
+    // take fewer, longer tiles (2^24 elements: 32 us instead of 38 per
+    // call); short ones keep more tiles in flight.  Either count is at most
+    // scan_tiles(n), the look-back words allocated.
+    if (n >= kScanLargeFrom) {
+        constexpr uint32_t kTile = kScanBlock * kScanItemsLarge;
+        const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+        hipLaunchKernelGGL((k_exclusive_scan<kScanBlock, kScanItemsLarge>), dim3(tiles), dim3(kScanBlock), 0, s, in,
+                           out, n, zero_input ? 1u : 0u, state, ticket, epoch, tiles, err);
+    } else {
+        const uint32_t tiles = scan_tiles(n);
+        hipLaunchKernelGGL((k_exclusive_scan<kScanBlock, kScanItems>), dim3(tiles), dim3(kScanBlock), 0, s, in, out,
+                           n, zero_input ? 1u : 0u, state, ticket, epoch, tiles, err);
+    }
     return hipGetLastError();
 }
 

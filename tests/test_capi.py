@@ -105,7 +105,7 @@ def test_error_codes_without_gpu(mtlp):
     assert L.mtlp_exclusive_scan(None, None, None, None, None, 10) == 1  # NULL buffers
     assert L.mtlp_exclusive_scan(None, None, 0x1004, 0x2000, 0x3000, 10) == 1  # misaligned
     assert L.mtlp_exclusive_scan(None, None, 0x1000, 0x2000, 0x3000, 0x80000000) == 2
-    assert L.mtlp_scan_scratch_bytes(4096) == 8 + 16 and L.mtlp_scan_scratch_bytes(4097) == 16 + 16
+    assert L.mtlp_scan_scratch_bytes(16384) == 8 + 16 and L.mtlp_scan_scratch_bytes(16385) == 16 + 16  # 16,384-element tiles
     # the allocation-free scan: limits checked before any device call
     assert L.mtlp_exclusive_scan_noalloc(None, None, None, 0) == 0              # n == 0: no-op
     assert L.mtlp_exclusive_scan_noalloc(None, 0x1000, 0x2000, (1 << 28) + 1) == 2  # above its maximum

@@ -47,7 +47,10 @@ SCAN_CASES = [("zeros", 100, 0, 0), ("ones", 500, 1, 1), ("rand", 1024, 0xA, 7),
               ("rand", 2048, 0xC, 7), ("rand", 635626, 0xD, 3), ("rand", 1, 1, 7), ("rand", 4095, 2, 7),
               ("rand", 4096, 3, 7), ("rand", 4097, 4, 7), ("rand", 530605, 5, 7),
               ("rand", (1 << 20) + 1, 6, 7), ("rand", 2097153, 7, 7), ("rand", (1 << 24) + 1, 8, 7),
-              ("big", 100000, 9, 0xFFFFFFFF)]
+              ("big", 100000, 9, 0xFFFFFFFF),
+              # tile edges (16,384 elements; 32,768 from 2^22 on) and the switch at 2^22
+              ("rand", 16383, 10, 7), ("rand", 16384, 11, 7), ("rand", 16385, 12, 7),
+              ("rand", (1 << 22) - 1, 13, 7), ("rand", 1 << 22, 14, 7), ("rand", (1 << 22) + 32769, 15, 7)]
 
 
 @pytest.mark.parametrize("kind,n,seed,maxv", SCAN_CASES)
