@@ -58,13 +58,19 @@ __global__ __launch_bounds__(kBlock) void k_scatter_ticket(
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t n = a.n;
     if (i >= n) return;
+    // every load that does not need the cell range goes out with the key,
+    // so the pass waits two memory latencies (key, then range) instead of
+    // one more for x and v after the range
     const uint32_t key = keys[i];
+    const uint32_t ticket = tickets[i];
+    const float4 xi = x[i];
+    const float4 vi = a.predict ? v[i] : xi;
     const CellRange r = cell_range(cell_start, key);
     const uint32_t lo = r.lo;
-    const uint32_t d = lo + tickets[i];
+    const uint32_t d = lo + ticket;
     if (d < n) {
         tmp[d] = make_uint2(i, key);
-        stage[d] = binned_pos(x, v, i, a);
+        stage[d] = binned_pos_of(xi, vi, i, a);
         // a huge cell's members, counted per index block in the cell's
         // first kHugeBlocks slots (i / block length < kHugeBlocks <= len)
         const uint32_t len = r.hi - lo;
@@ -141,11 +147,17 @@ __global__ __launch_bounds__(kBlock) void k_rank_gather(
         if (s == lo) big_list[atomicAdd(big_count, 1u)] = key;
         return;
     }
-    uint32_t rank = 0;
-    for (uint32_t m = lo; m < hi; ++m) rank += (tmp[m].x < i) ? 1u : 0u;
+    // the slot's staged position is read before the ranking, and the cell's
+    // indices four at a time, so their loads overlap
+    const float4 p = stage[s];
+    uint32_t rank = 0, m = lo;
+    for (; m + 4u <= hi; m += 4u) {
+        const uint32_t a0 = tmp[m].x, a1 = tmp[m + 1u].x, a2 = tmp[m + 2u].x, a3 = tmp[m + 3u].x;
+        rank += (a0 < i ? 1u : 0u) + (a1 < i ? 1u : 0u) + (a2 < i ? 1u : 0u) + (a3 < i ? 1u : 0u);
+    }
+    for (; m < hi; ++m) rank += (tmp[m].x < i) ? 1u : 0u;
     const uint32_t dst = lo + rank;
     sorted_index[dst] = i;
-    const float4 p = stage[s];
     sorted[dst] = p;
     store_xyz(xyz, dst, p);
     if (slot_of) slot_of[i] = dst;

@@ -119,14 +119,16 @@ __device__ __forceinline__ float4 predict_pos(float4 x, float4 v, float gx, floa
 
 // Position particle i is binned at: x* on the step path, x for a plain
 // spatial-hash build.
-__device__ __forceinline__ float4 binned_pos(const float4* __restrict__ x, const float4* __restrict__ v,
-                                             uint32_t i, const StepArgs& a) {
-    const float4 xi = x[i];
+__device__ __forceinline__ float4 binned_pos_of(const float4& xi, const float4& vi, uint32_t i, const StepArgs& a) {
     if (!a.predict) return xi;
-    const float4 vi = v[i];
     float4 out;
     with_scene(i, a, [&](const SceneDev& sc) { out = predict_pos(xi, vi, sc.gx, sc.gy, sc.gz, a.dt); });
     return out;
+}
+__device__ __forceinline__ float4 binned_pos(const float4* __restrict__ x, const float4* __restrict__ v,
+                                             uint32_t i, const StepArgs& a) {
+    const float4 xi = x[i];
+    return binned_pos_of(xi, a.predict ? v[i] : xi, i, a);
 }
 
 // Slot s of a packed (x, y, z) position copy (12 bytes a slot; NULL: none).
