@@ -17,7 +17,7 @@
 //     the scatter's stores rather than gathers.  Cells larger than
 //     kSmallCell are listed and sorted by a separate kernel (LDS bitonic
 //     sort; past kLdsSortMax a rank per index block, see kHugeBlocks).
-#include "mtlp_device.h"
+#include "mtlp_stencil.h"
 
 namespace mtlp {
 
@@ -270,30 +270,19 @@ __global__ __launch_bounds__(kBigBlock) void k_big_cells(
     }
 }
 
+// The neighbour count of slot s: the set bits of its stencil chunks, by the
+// density pass's filter (eight candidate loads in flight per group, the
+// same neighbour test), so the counts equal the iteration-0 density pass's.
+// Reads up to kChunkSlack slots past the last particle, like that filter.
 __device__ __forceinline__ void count_neighbors_slot(uint32_t s, const SceneDev& sc,
                                                      const float4* __restrict__ sorted,
                                                      const uint32_t* __restrict__ cell_start,
                                                      const uint32_t* __restrict__ sorted_index,
                                                      uint32_t n, uint32_t* __restrict__ nbr) {
-    const GridDev g = sc.g;
-    const float r2max = sc.sd.h2;  // neighborRadiusSq = kSmoothingH^2, Engine.cpp:447
     const float4 p = sorted[s];
-    const Cell3 c = cell_of(p.x, p.y, p.z, g);
-    const int x0 = max(c.x - 1, 0), x1 = min(c.x + 1, (int)g.dx - 1);
-    const int y0 = max(c.y - 1, 0), y1 = min(c.y + 1, (int)g.dy - 1);
-    const int z0 = max(c.z - 1, 0), z1 = min(c.z + 1, (int)g.dz - 1);
     uint32_t cnt = 0;
-    for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y) {
-            const uint32_t row = sc.cell_off + (uint32_t)y * g.dx + (uint32_t)z * g.dxy;
-            const uint32_t lo = cell_start[row + (uint32_t)x0];
-            const uint32_t hi = min(cell_start[row + (uint32_t)x1 + 1], n);
-            for (uint32_t j = lo; j < hi; ++j) {
-                const float4 q = sorted[j];
-                const float d = nbr_key(q.x - p.x, q.y - p.y, q.z - p.z, r2max);
-                cnt += (j != s && d < 0.0f) ? 1u : 0u;
-            }
-        }
+    for_each_chunk<true>(s, p, sc, sorted, cell_start, n,
+                         [&](uint32_t, uint32_t mask) { cnt += (uint32_t)__builtin_popcount(mask); });
     nbr[sorted_index[s]] = cnt;
 }
 
