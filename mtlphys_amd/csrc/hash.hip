@@ -299,8 +299,6 @@ __global__ __launch_bounds__(kBlock) void k_count_neighbors(
     with_scene(s, a, [&](const SceneDev& sc) { count_neighbors_slot(s, sc, sorted, cell_start, sorted_index, n, nbr); });
 }
 
-inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
-
 }  // namespace
 
 hipError_t launch_predict_hash_count(const HashBufs& b, const StepArgs& a, hipStream_t s) {
