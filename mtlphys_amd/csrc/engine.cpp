@@ -191,7 +191,6 @@ mtlp_status alloc_cells(mtlp_engine* e, uint32_t C) {
     HIP_TRY(hipMemset(e->counts, 0, sizeof(uint32_t) * ((size_t)C + 1)));
     HIP_TRY(hipMemset(e->cell_start, 0, sizeof(uint32_t) * ((size_t)C + 1)));
     HIP_TRY(hipMemset(e->scan_state, 0, sizeof(uint64_t) * tiles));
-    e->scan_epoch = 1;  // eager launches use epochs >= 2; graph replays use 1
     e->cells = C;
     return MTLP_OK;
 }
@@ -299,23 +298,15 @@ namespace mtlp {
 
 // Engine.cpp:601-667 and :715-726.
 mtlp_status enqueue_hash(mtlp_engine* e, hipStream_t s, const StepArgs& a, StageClock& clk, const HashBufs& b) {
-    HIP_TRY(launch_predict_hash_count(b, a, s));
+    // the predict pass clears the scan's look-back words, which the scan
+    // then uses with epoch 1 (the same in eager launches and graph replays)
+    HashBufs hb = b;
+    hb.scan_words = reinterpret_cast<uint32_t*>(e->scan_state);
+    hb.n_scan_words = 2 * scan_tiles(e->cells + 1);
+    HIP_TRY(launch_predict_hash_count(hb, a, s));
     clk.mark("predict_hash_count");
-    uint32_t epoch;
-    const uint32_t tiles = scan_tiles(e->cells + 1);
-    if (e->capturing) {
-        // a replayed graph reuses its epoch: clear the look-back words first
-        HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(e->scan_state), 0u, 2 * (size_t)tiles, s));
-        epoch = 1;
-    } else {
-        if (e->scan_epoch >= kEpochLimit) {
-            HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(e->scan_state), 0u, 2 * (size_t)tiles, s));
-            e->scan_epoch = 1;  // epoch 1 is what graph replays use
-        }
-        epoch = ++e->scan_epoch;
-    }
     HIP_TRY(launch_exclusive_scan(b.counts, b.cell_start, e->cells + 1, true, e->scan_state,
-                                  e->misc + 1, epoch, e->misc + 2, s));
+                                  e->misc + 1, 1u, e->misc + 2, s));
     clk.mark("exclusive_scan");
     HIP_TRY(launch_scatter_gather(b, a, s));
     clk.mark("stable_scatter_gather");

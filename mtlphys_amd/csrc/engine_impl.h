@@ -45,7 +45,6 @@ struct DeviceGuard {
 constexpr uint32_t kPad = 256;  // particle buffers padded like Engine.cpp:383
 constexpr uint32_t kMaxIterations = 256;
 constexpr uint32_t kMaxScorrN = 64;
-constexpr uint32_t kEpochLimit = (1u << 30) - 1u;
 constexpr uint32_t kMaxScenes = 1u << 20;
 
 inline uint32_t round_up(uint32_t n, uint32_t m) { return ((n + m - 1) / m) * m; }
@@ -195,7 +194,6 @@ struct mtlp_engine {
     uint32_t* counts = nullptr;
     uint32_t* cell_start = nullptr;
     uint64_t* scan_state = nullptr;
-    uint32_t scan_epoch = 0;
     // misc device words: [0] big-cell count, [1] scan ticket, [2] error
     // flags, [3] huge-cell flag
     uint32_t* misc = nullptr;

@@ -26,8 +26,13 @@ namespace {
 __global__ __launch_bounds__(kBlock) void k_predict_hash_count(
     const float4* __restrict__ x, const float4* __restrict__ v, StepArgs a,
     uint32_t* __restrict__ keys, uint32_t* __restrict__ tickets, uint32_t* __restrict__ counts,
-    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ huge_flag) {
+    uint32_t* __restrict__ big_count, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ huge_flag,
+    uint32_t* __restrict__ scan_words, uint32_t n_scan_words) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    // the scan that follows starts from cleared look-back words: every build
+    // scans with the same epoch, so a replayed graph needs no clearing
+    // launch and no host-side epoch
+    for (uint32_t j = i; j < n_scan_words; j += gridDim.x * kBlock) scan_words[j] = 0u;
     if (i == 0) {  // the big-cell list of this build starts empty, with no huge cell
         *big_count = 0u;
         *huge_flag = 0u;
@@ -304,7 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_count_neighbors(
 hipError_t launch_predict_hash_count(const HashBufs& b, const StepArgs& a, hipStream_t s) {
     const uint32_t grid = blocks_for(a.n > 0 ? a.n : 1, kBlock);
     hipLaunchKernelGGL(k_predict_hash_count, dim3(grid), dim3(kBlock), 0, s, b.x, b.v, a, b.keys,
-                       b.tickets, b.counts, b.big_count, b.heavy_count, b.huge_flag);
+                       b.tickets, b.counts, b.big_count, b.heavy_count, b.huge_flag, b.scan_words,
+                       b.n_scan_words);
     return hipGetLastError();
 }
 

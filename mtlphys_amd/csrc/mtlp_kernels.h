@@ -128,6 +128,8 @@ struct HashBufs {
     uint32_t* hcnt = nullptr;      // n: huge cells' member counts per index block, zero between builds
     uint32_t* hoff = nullptr;      // n: their starting ranks per index block
     uint32_t* huge_flag = nullptr; // set when this build has a huge cell
+    uint32_t* scan_words = nullptr; // look-back words of the scan that follows (cleared by the predict pass)
+    uint32_t n_scan_words = 0;
 };
 
 // ---- launchers (return hipGetLastError after the launches) -------------

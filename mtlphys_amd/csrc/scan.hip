@@ -16,9 +16,10 @@
 //     already run; wave 0 inspects 64 predecessors per step.
 //
 // Look-back words are 64 bit: [63:34] epoch, [33:32] status (1 aggregate,
-// 2 inclusive prefix), [31:0] value.  The host bumps the epoch on every
-// launch, so words left by earlier launches read as "not ready" and the
-// state never needs clearing between launches.
+// 2 inclusive prefix), [31:0] value; a word of another epoch reads as "not
+// ready".  Every caller clears the words before a launch and passes epoch 1:
+// the scan entry points with a fill kernel, the engine's hash build in the
+// predict pass before its scan (so a replayed graph needs no extra launch).
 #include <mutex>
 
 #include "mtlp_device.h"
