@@ -64,6 +64,7 @@ __global__ __launch_bounds__(kB) void k_exclusive_scan(
     uint32_t* in, uint32_t* out, uint32_t n, uint32_t zero_input,
     uint64_t* __restrict__ state, uint32_t* __restrict__ ticket, uint32_t epoch,
     uint32_t num_tiles, uint32_t* __restrict__ err) {
+    static_assert(kB % 64 == 0 && kItems % 4 == 0 && kItems / 4 <= kB, "whole waves, uint4 columns, a thread per column");
     constexpr int kVec = (int)(kItems / 4);  // uint4 per thread
     constexpr uint32_t kWaves = kB / 64;
     constexpr size_t kTile = (size_t)kB * kItems;
